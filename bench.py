@@ -1,0 +1,182 @@
+"""Benchmark: Block Vecchia FP64 log-likelihood evaluations per second on one
+MI355X, at the reference's performance shape (README.md:40-46):
+
+  n = 20,000 locations iid U[0,1]^2 (seed 0); Matern sigma^2 = 1.5,
+  range = 0.1, nu = 0.5, no nugget; observations y = L z with L the dense
+  FP64 Cholesky of the full covariance; k-means into 1,500 blocks; cs = 300
+  nearest conditioning points per block.
+
+A step is one full likelihood evaluation (covariance generation, per-block
+factorization, log-determinant and quadratic form, reduction, the 8-byte
+result back on the host) with the dataset resident in HBM.  Prints one JSON
+line with the driver's contract fields plus `roofline` (the factorization
+kernel's FP64 rate, HIP-event timed) and `cpu_baseline` (the CPU port of the
+same chain, timed on this host's cores).
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Vecchia llh evals/s + p50/p99 latency at 1e-10 rel err; FP64 TFLOP/s vs peak"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def make_workload(n=20000, bc=1500, cs=300, theta=(1.5, 0.1, 0.5), seed=0):
+    import blockvecchia_amd as bv
+    from blockvecchia_amd import datagen
+    x, y = datagen.uniform_locations(n, seed)
+    t0 = time.time()
+    try:
+        import torch
+        dev = "cuda" if torch.cuda.is_available() else None
+    except Exception:
+        dev = None
+    obs = datagen.sample_dense(x, y, theta[0], theta[1], theta[2], seed=seed, device=dev)
+    log("observations (dense Cholesky sampling, %s): %.1f s" % (dev or "cpu", time.time() - t0))
+    t0 = time.time()
+    blocks = bv.partition(x, y, obs, bc, cs, seed=seed)
+    log("partition (k-means + kNN): %.1f s, %d blocks" % (time.time() - t0, blocks.batch_count))
+    return blocks
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the factorization kernel from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
+    calibration), or None."""
+    p = os.path.join(ROOT, "profiles", "bv_block_kernel_n20000_cs300_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-repeats", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the likelihood shards by blocks "
+              "across GPUs with a single 8-byte sum; see DESIGN.md); --gpus %d is not run"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import blockvecchia_amd as bv
+    from blockvecchia_amd import flops
+
+    n, bc, cs = 20000, 1500, 300
+    theta = [1.5, 0.1, 0.5]
+    blocks = make_workload(n, bc, cs, theta)
+    F = flops.eval_flops(blocks.batch_num, cs)
+    eng = bv.Engine(blocks)
+    for _ in range(args.warmup):
+        eng.llh(theta)
+    st, bad = eng.status()
+    if st != bv.BV_OK:
+        raise SystemExit("evaluation failed: status %d block %d" % (st, bad))
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        torch = None
+    lat = []
+    kms = []
+    llh = None
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        llh = eng.llh(theta)
+        lat.append(time.perf_counter() - t0)
+        kms.append(eng.last_kernel_ms())
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    evals_per_s = args.steps / elapsed
+    lat_ms = np.array(lat) * 1e3
+    kern_ms = float(np.mean(kms))
+    achieved = F / (kern_ms * 1e-3) / 1e12
+    log("llh %.17g  %.3f ms/eval  kernel %.3f ms  %.2f TFLOP/s" %
+        (llh, 1e3 * elapsed / args.steps, kern_ms, achieved))
+
+    cpu = None
+    rel_err = None
+    if not args.no_cpu_baseline:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        oracle.llh(blocks, theta, threads=threads)  # warm-up
+        ts = []
+        ov = None
+        for _ in range(args.cpu_repeats):
+            t0 = time.perf_counter()
+            ov = oracle.llh(blocks, theta, threads=threads)
+            ts.append(time.perf_counter() - t0)
+        med = float(np.median(ts))
+        rel_err = abs(llh - ov) / abs(ov)
+        cpu = {"value": 1.0 / med, "unit": "evals/s", "cores": threads, "kind": "port",
+               "sample": "the full n=20000 evaluation (all %d blocks), median of %d after one "
+                         "warm-up; CPU restatement of the reference chain (oracle/oracle.c, "
+                         "OpenMP over blocks)" % (blocks.batch_count, args.cpu_repeats)}
+        log("cpu port %.3f s/eval on %d threads, gpu vs cpu rel err %.3e" % (med, threads, rel_err))
+
+    traffic = traffic_from_profiles()
+    out = {
+        "metric": METRIC,
+        "value": evals_per_s,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "n=20000 iid U[0,1]^2 seed 0, Matern sigma2=1.5 range=0.1 nu=0.5, "
+                        "y from dense FP64 Cholesky, k-means bc=1500, cs=300",
+            "n": n, "bc": bc, "cs": cs, "batch_count": int(blocks.batch_count),
+            "parallelism": "single-gpu",
+        },
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
+                       "p99": float(np.percentile(lat_ms, 99))},
+        "llh": llh,
+        "rel_err_vs_cpu": rel_err,
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": traffic,
+            "kernel": "bv_block_kernel (nu=1/2 closed form)",
+            "flops_per_launch": F,
+            "kernel_ms": kern_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

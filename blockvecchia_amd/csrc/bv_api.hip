@@ -1,0 +1,483 @@
+/*
+ * bv_api.hip -- host side of libbv: dataset upload, evaluation launches and
+ * the C ABI of include/bv.h.
+ *
+ * One bv_ctx holds one dataset resident in HBM (block-ordered locations,
+ * observations, block sizes, conditioning indices or copies), the launch
+ * descriptors of its blocks, and a workspace for the factor tiles of every
+ * block of up to `cap_evals` simultaneous evaluations.  An evaluation is two
+ * launches on the context's stream (bv_block_kernel over all blocks, then
+ * bv_reduce_kernel) and one 32-byte device-to-host copy of the result.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+
+#include "bv.h"
+#include "bv_engine.h"
+
+hipError_t bv_launch_block(int kind, unsigned nblk, size_t lds, hipStream_t st,
+                           const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
+hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,
+                         const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
+hipError_t bv_launch_reduce(unsigned nevals, hipStream_t st, const bv_eval_slot *slots);
+hipError_t bv_launch_matern(int m, int n, const double *x1, const double *y1, const double *z1,
+                            const double *x2, const double *y2, const double *z2,
+                            const bv_kparams &kp, int kind, double *A);
+hipError_t bv_launch_selftest(const double *a, const double *b, double *d);
+
+#define BV_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            fprintf(stderr, "libbv: %s failed at %s:%d: %s\n", #call, __FILE__, __LINE__, \
+                    hipGetErrorString(e_));                                             \
+            return BV_ERR_HIP;                                                          \
+        }                                                                               \
+    } while (0)
+
+struct bv_ctx {
+    int device = 0;
+    int64_t n = 0;
+    int32_t B = 0, cs = 0, kernel = 1, metric = 0, time_flag = 0, verbose = 0;
+    double dist_scale = 1.0;
+    std::vector<int32_t> h_batch_num;
+    std::vector<int64_t> h_accum;
+    /* device dataset */
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_obs = nullptr;
+    int32_t *d_nbr = nullptr;
+    double *d_cx = nullptr, *d_cy = nullptr, *d_cz = nullptr, *d_cobs = nullptr;
+    int32_t *d_batch_num = nullptr;
+    /* launch layout of one evaluation */
+    std::vector<bv_block_desc> base_desc;   /* sorted by padded size, descending */
+    int64_t ws_per_eval = 0;                /* doubles                            */
+    int32_t max_mp = 0;
+    /* per-capacity buffers */
+    int cap_evals = 0;
+    bv_block_desc *d_desc = nullptr;
+    int desc_evals = 0;                     /* evaluations d_desc is laid out for */
+    bv_eval_slot *d_slots = nullptr, *h_slots = nullptr;
+    double *d_ws = nullptr;
+    double *d_logdet = nullptr, *d_dot = nullptr, *d_result = nullptr, *h_result = nullptr;
+    int32_t *d_info = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    /* state of the last evaluation */
+    int last_status = BV_OK;
+    int last_bad = -1;
+    int last_k = 0;
+    double last_ms = 0.0;
+    int iterations = 0;
+};
+
+static int bv_nparams(int kernel) {
+    return kernel == BV_KERNEL_SPACETIME ? 7 : (kernel == BV_KERNEL_POWEXP_NUGGET ? 4 : 3);
+}
+
+template <class T>
+static int bv_upload(T **dst, const T *src, size_t count, hipStream_t s) {
+    if (count == 0) { *dst = nullptr; return BV_OK; }
+    BV_HIP(hipMalloc((void **)dst, count * sizeof(T)));
+    BV_HIP(hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return BV_OK;
+}
+
+static void bv_free_eval_buffers(bv_ctx *c) {
+    if (c->d_desc) hipFree(c->d_desc);
+    if (c->d_slots) hipFree(c->d_slots);
+    if (c->h_slots) hipHostFree(c->h_slots);
+    if (c->d_ws) hipFree(c->d_ws);
+    if (c->d_logdet) hipFree(c->d_logdet);
+    if (c->d_dot) hipFree(c->d_dot);
+    if (c->d_info) hipFree(c->d_info);
+    if (c->d_result) hipFree(c->d_result);
+    if (c->h_result) hipHostFree(c->h_result);
+    c->d_desc = nullptr; c->d_slots = nullptr; c->h_slots = nullptr; c->d_ws = nullptr;
+    c->d_logdet = nullptr; c->d_dot = nullptr; c->d_info = nullptr;
+    c->d_result = nullptr; c->h_result = nullptr;
+    c->cap_evals = 0; c->desc_evals = 0;
+}
+
+/* Make room for k simultaneous evaluations. */
+static int bv_reserve(bv_ctx *c, int k) {
+    if (k <= c->cap_evals) return BV_OK;
+    bv_free_eval_buffers(c);
+    const size_t B = (size_t)c->B;
+    BV_HIP(hipMalloc((void **)&c->d_desc, sizeof(bv_block_desc) * B * k));
+    BV_HIP(hipMalloc((void **)&c->d_slots, sizeof(bv_eval_slot) * k));
+    BV_HIP(hipHostMalloc((void **)&c->h_slots, sizeof(bv_eval_slot) * k, hipHostMallocDefault));
+    BV_HIP(hipMalloc((void **)&c->d_ws, sizeof(double) * (size_t)c->ws_per_eval * k));
+    BV_HIP(hipMalloc((void **)&c->d_logdet, sizeof(double) * B * k));
+    BV_HIP(hipMalloc((void **)&c->d_dot, sizeof(double) * B * k));
+    BV_HIP(hipMalloc((void **)&c->d_info, sizeof(int32_t) * B * k));
+    BV_HIP(hipMalloc((void **)&c->d_result, sizeof(double) * 4 * k));
+    BV_HIP(hipHostMalloc((void **)&c->h_result, sizeof(double) * 4 * k, hipHostMallocDefault));
+    c->cap_evals = k;
+    return BV_OK;
+}
+
+/* Lay out the descriptors of k evaluations, interleaved so that the largest
+   blocks of every evaluation start first. */
+static int bv_layout(bv_ctx *c, int k) {
+    if (c->desc_evals == k) return BV_OK;
+    std::vector<bv_block_desc> all;
+    all.reserve((size_t)c->B * k);
+    for (const bv_block_desc &d0 : c->base_desc)
+        for (int e = 0; e < k; ++e) {
+            bv_block_desc d = d0;
+            d.eval = e;
+            d.ws_off = d0.ws_off + (int64_t)e * c->ws_per_eval;
+            all.push_back(d);
+        }
+    BV_HIP(hipMemcpyAsync(c->d_desc, all.data(), sizeof(bv_block_desc) * all.size(),
+                          hipMemcpyHostToDevice, c->stream));
+    BV_HIP(hipStreamSynchronize(c->stream));
+    c->desc_evals = k;
+    return BV_OK;
+}
+
+/* Host preparation of the covariance parameters, as core_dcmg computes its
+   constant (src/ckernel.cpp:1070-1074). */
+static int bv_make_kparams(const bv_ctx *c, const double *theta, bv_kparams *kp, int *kind) {
+    const int np = bv_nparams(c->kernel);
+    for (int i = 0; i < np; ++i)
+        if (!isfinite(theta[i])) return BV_ERR_ARG;
+    memset(kp, 0, sizeof(*kp));
+    kp->sigma2 = theta[0];
+    kp->beta = theta[1];
+    kp->nu = theta[2];
+    double con = pow(2, (theta[2] - 1)) * tgamma(theta[2]);
+    con = 1.0 / con;
+    kp->con = theta[0] * con;
+    kp->dist_scale = c->dist_scale;
+    kp->metric = c->metric;
+    kp->use_z = (c->time_flag && c->d_z) ? 1 : 0;
+    /* the reference's box bounds are < 100 (include/vecchia_helper_data.h:684-700);
+       the K_nu order recurrence runs nu + 1/2 steps */
+    if (!(theta[2] <= 100.0)) return BV_ERR_ARG;
+    if (c->kernel == BV_KERNEL_SPACETIME) {
+        kp->beta_t = theta[3]; kp->nu_t = theta[4]; kp->sep = theta[5]; kp->aux = theta[6];
+        *kind = BV_KIND_SPACETIME;
+        if (!c->d_z) return BV_ERR_ARG;
+        return BV_OK;
+    }
+    if (!(theta[1] > 0.0) || !(theta[2] > 0.0)) return BV_ERR_ARG;
+    if (theta[2] == 0.5) *kind = BV_KIND_NU05;
+    else if (theta[2] == 1.5) *kind = BV_KIND_NU15;
+    else if (theta[2] == 2.5) *kind = BV_KIND_NU25;
+    else *kind = BV_KIND_GENERAL;
+    return BV_OK;
+}
+
+extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
+    if (!p || !out) return BV_ERR_ARG;
+    *out = nullptr;
+    if (p->n <= 0 || p->batch_count <= 0 || p->cs < 0 || !p->batch_num || !p->x || !p->y ||
+        !p->obs)
+        return BV_ERR_ARG;
+    if (p->kernel < 1 || p->kernel > 4) return BV_ERR_ARG;
+    if ((p->time_flag || p->kernel == BV_KERNEL_SPACETIME) && !p->z) return BV_ERR_ARG;
+    if (p->distance_metric == 1 && p->time_flag) return BV_ERR_UNSUPPORTED;
+    if (p->cs > 0 && p->batch_count > 1 && !p->nbr && !(p->con_x && p->con_y && p->con_obs))
+        return BV_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || p->device >= ndev)
+        return BV_ERR_NO_DEVICE;
+    BV_HIP(hipSetDevice(p->device));
+
+    bv_ctx *c = new bv_ctx();
+    c->device = p->device;
+    c->n = p->n;
+    c->B = p->batch_count;
+    c->cs = p->cs;
+    c->kernel = p->kernel;
+    c->metric = p->distance_metric;
+    c->time_flag = p->time_flag || p->kernel == BV_KERNEL_SPACETIME;
+    c->dist_scale = p->dist_scale > 0 ? p->dist_scale : 1.0;
+    c->verbose = p->verbose;
+    c->h_batch_num.assign(p->batch_num, p->batch_num + p->batch_count);
+    c->h_accum.assign((size_t)c->B + 1, 0);
+    for (int i = 0; i < c->B; ++i) {
+        if (p->batch_num[i] <= 0) { delete c; return BV_ERR_ARG; }
+        c->h_accum[i + 1] = c->h_accum[i] + p->batch_num[i];
+    }
+    if (c->h_accum[c->B] != c->n) { delete c; return BV_ERR_ARG; }
+    if (p->nbr && c->cs > 0) {
+        for (int64_t i = (int64_t)c->cs; i < (int64_t)c->B * c->cs; ++i)
+            if (p->nbr[i] < 0 || p->nbr[i] >= c->n) { delete c; return BV_ERR_ARG; }
+    }
+
+    /* launch descriptors: block sizes, padding, workspace offsets */
+    std::vector<bv_block_desc> desc((size_t)c->B);
+    int64_t off = 0;
+    for (int i = 0; i < c->B; ++i) {
+        bv_block_desc &d = desc[i];
+        memset(&d, 0, sizeof(d));
+        d.block = i;
+        d.cs0 = (i == 0 || c->cs == 0) ? 0 : c->cs;
+        d.m = d.cs0 + p->batch_num[i];
+        d.mp = ((d.m + 1 + BV_NB - 1) / BV_NB) * BV_NB;
+        d.obs_off = c->h_accum[i];
+        const int64_t T = d.mp / BV_TILE;
+        d.ws_off = off;
+        off += T * (T + 1) / 2 * 256;
+        c->max_mp = std::max(c->max_mp, d.mp);
+    }
+    if (c->max_mp > BV_MAX_MP) { delete c; return BV_ERR_UNSUPPORTED; }
+    c->ws_per_eval = off;
+    std::stable_sort(desc.begin(), desc.end(),
+                     [](const bv_block_desc &a, const bv_block_desc &b) { return a.mp > b.mp; });
+    c->base_desc = desc;
+
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        bv_destroy(c);
+        return BV_ERR_HIP;
+    }
+    int rc = BV_OK;
+    const size_t n = (size_t)c->n;
+    if ((rc = bv_upload(&c->d_x, p->x, n, c->stream)) ||
+        (rc = bv_upload(&c->d_y, p->y, n, c->stream)) ||
+        (rc = bv_upload(&c->d_obs, p->obs, n, c->stream)) ||
+        (rc = bv_upload(&c->d_batch_num, p->batch_num, (size_t)c->B, c->stream))) {
+        bv_destroy(c);
+        return rc;
+    }
+    if (c->time_flag && (rc = bv_upload(&c->d_z, p->z, n, c->stream))) { bv_destroy(c); return rc; }
+    const size_t ncon = (size_t)c->B * c->cs;
+    if (c->cs > 0) {
+        if (p->nbr) {
+            if ((rc = bv_upload(&c->d_nbr, p->nbr, ncon, c->stream))) { bv_destroy(c); return rc; }
+        } else {
+            if ((rc = bv_upload(&c->d_cx, p->con_x, ncon, c->stream)) ||
+                (rc = bv_upload(&c->d_cy, p->con_y, ncon, c->stream)) ||
+                (rc = bv_upload(&c->d_cobs, p->con_obs, ncon, c->stream))) {
+                bv_destroy(c);
+                return rc;
+            }
+            if (c->time_flag && (rc = bv_upload(&c->d_cz, p->con_z, ncon, c->stream))) {
+                bv_destroy(c);
+                return rc;
+            }
+        }
+    }
+    if ((rc = bv_reserve(c, 1)) || (rc = bv_layout(c, 1))) { bv_destroy(c); return rc; }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { bv_destroy(c); return BV_ERR_HIP; }
+    *out = c;
+    return BV_OK;
+}
+
+/* Launch k evaluations (thetas row-major, nparam each) and wait for them. */
+static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *out, int *status) {
+    if (k <= 0 || !thetas || !out) return BV_ERR_ARG;
+    if (nparam != bv_nparams(c->kernel)) return BV_ERR_ARG;
+    BV_HIP(hipSetDevice(c->device));
+    int rc;
+    if ((rc = bv_reserve(c, k)) || (rc = bv_layout(c, k))) return rc;
+    const size_t B = (size_t)c->B;
+    for (int e = 0; e < k; ++e) {
+        bv_eval_slot &s = c->h_slots[e];
+        memset(&s, 0, sizeof(s));
+        if ((rc = bv_make_kparams(c, thetas + (size_t)e * nparam, &s.kp, &s.kind))) return rc;
+        s.x = c->d_x; s.y = c->d_y; s.z = c->d_z; s.obs = c->d_obs;
+        s.nbr = c->d_nbr; s.cx = c->d_cx; s.cy = c->d_cy; s.cz = c->d_cz; s.cobs = c->d_cobs;
+        s.cs = c->cs;
+        s.nblocks = c->B;
+        s.batch_num = c->d_batch_num;
+        s.logdet = c->d_logdet + e * B;
+        s.dot = c->d_dot + e * B;
+        s.info = c->d_info + e * B;
+        s.result = c->d_result + 4 * e;
+        s.log2pi = log(2 * 3.141592653589793);
+    }
+    const size_t lds = sizeof(double) * 4 * (size_t)c->max_mp;
+    BV_HIP(hipEventRecord(c->ev0, c->stream));
+    /* one kind per launch: evaluations of a call share the kernel when their
+       smoothness matches; otherwise one launch per kind over its own slots */
+    int kinds_used = 0;
+    for (int e = 0; e < k; ++e) kinds_used |= 1 << c->h_slots[e].kind;
+    if (kinds_used & (kinds_used - 1)) {
+        /* smoothness differs across the call: all go through the general path */
+        for (int e = 0; e < k; ++e) c->h_slots[e].kind = BV_KIND_GENERAL;
+        kinds_used = 1 << BV_KIND_GENERAL;
+    }
+    BV_HIP(hipMemcpyAsync(c->d_slots, c->h_slots, sizeof(bv_eval_slot) * k,
+                          hipMemcpyHostToDevice, c->stream));
+    const unsigned nblk = (unsigned)(B * k);
+    for (int kind = 0; kind < 8; ++kind) {
+        if (!(kinds_used & (1 << kind))) continue;
+        if (kind == BV_KIND_GENERAL || kind == BV_KIND_SPACETIME) {
+            const unsigned T = (unsigned)(c->max_mp / BV_TILE);
+            BV_HIP(bv_launch_gen(kind, T * (T + 1) / 2, nblk, c->stream, c->d_desc, c->d_slots,
+                                 c->d_ws));
+        }
+        BV_HIP(bv_launch_block(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
+    }
+    BV_HIP(hipEventRecord(c->ev1, c->stream));
+    BV_HIP(bv_launch_reduce((unsigned)k, c->stream, c->d_slots));
+    BV_HIP(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * 4 * k,
+                          hipMemcpyDeviceToHost, c->stream));
+    BV_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    c->last_k = k;
+    int worst = BV_OK;
+    c->last_bad = -1;
+    for (int e = 0; e < k; ++e) {
+        const double llh = c->h_result[4 * e];
+        const int bad = (int)c->h_result[4 * e + 1];
+        int st = BV_OK;
+        if (bad >= 0) {
+            st = BV_ERR_NOT_PD;
+            if (c->last_bad < 0) c->last_bad = bad;
+            out[e] = -INFINITY;
+        } else {
+            out[e] = llh;
+        }
+        if (status) status[e] = st;
+        if (st != BV_OK) worst = st;
+    }
+    c->last_status = worst;
+    return BV_OK;
+}
+
+extern "C" double bv_llh(unsigned n, const double *theta, double *grad, void *f_data) {
+    (void)grad;
+    bv_ctx *c = (bv_ctx *)f_data;
+    if (!c || !theta || (int)n != bv_nparams(c->kernel)) return NAN;
+    double llh = NAN;
+    int st = 0;
+    const int rc = bv_run(c, 1, (int)n, theta, &llh, &st);
+    if (rc != BV_OK) { c->last_status = rc; return NAN; }
+    if (c->verbose) {
+        if (c->kernel == 1 || c->kernel == 2)
+            printf("%dth Model Parameters (Variance, range, smoothness): (%1.8lf, %1.8lf, %1.8lf) -> Loglik: %.18lf \n",
+                   c->iterations, theta[0], theta[1], theta[2], llh);
+        else if (c->kernel == 3)
+            printf("%dth Model Parameters (Variance, range, smoothness, nugget): (%1.8lf, %1.8lf, %1.8lf, %1.8lf) -> Loglik: %.18lf \n",
+                   c->iterations, theta[0], theta[1], theta[2], theta[3], llh);
+        else
+            printf("%dth Model Parameters (Variance, beta, nu, beta_time, nu_time, sep): (%lf, %lf, %lf, %lf, %lf, %lf) -> Loglik: %lf \n",
+                   c->iterations, theta[0], theta[1], theta[2], theta[3], theta[4], theta[5], llh);
+    }
+    c->iterations += 1;
+    return llh;
+}
+
+extern "C" double llh_Xvecchia_batch(unsigned n, const double *theta, double *grad, void *f_data) {
+    return bv_llh(n, theta, grad, f_data);
+}
+
+extern "C" int bv_llh_many(bv_ctx *c, int k, int nparam, const double *thetas, double *out,
+                           int *status) {
+    if (!c) return BV_ERR_ARG;
+    const int rc = bv_run(c, k, nparam, thetas, out, status);
+    if (rc == BV_OK) c->iterations += k;
+    else c->last_status = rc;
+    return rc;
+}
+
+extern "C" int bv_block_terms(bv_ctx *c, double *logdet, double *dot, int32_t *info) {
+    if (!c || c->last_k <= 0) return BV_ERR_ARG;
+    BV_HIP(hipSetDevice(c->device));
+    const size_t B = (size_t)c->B;
+    if (logdet) BV_HIP(hipMemcpy(logdet, c->d_logdet, sizeof(double) * B, hipMemcpyDeviceToHost));
+    if (dot) BV_HIP(hipMemcpy(dot, c->d_dot, sizeof(double) * B, hipMemcpyDeviceToHost));
+    if (info) BV_HIP(hipMemcpy(info, c->d_info, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    return BV_OK;
+}
+
+extern "C" int bv_last_status(bv_ctx *c, int *first_bad_block) {
+    if (!c) return BV_ERR_ARG;
+    if (first_bad_block) *first_bad_block = c->last_bad;
+    return c->last_status;
+}
+
+extern "C" double bv_last_kernel_ms(bv_ctx *c) { return c ? c->last_ms : -1.0; }
+
+extern "C" int bv_iterations(bv_ctx *c) { return c ? c->iterations : -1; }
+
+extern "C" void bv_destroy(bv_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    bv_free_eval_buffers(c);
+    double *bufs[] = {c->d_x, c->d_y, c->d_z, c->d_obs, c->d_cx, c->d_cy, c->d_cz, c->d_cobs};
+    for (double *b : bufs)
+        if (b) hipFree(b);
+    if (c->d_nbr) hipFree(c->d_nbr);
+    if (c->d_batch_num) hipFree(c->d_batch_num);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int bv_matern(int m, int nn, const double *x1, const double *y1, const double *z1,
+                         const double *x2, const double *y2, const double *z2,
+                         const double *theta, int kernel, int distance_metric, double dist_scale,
+                         double *A, int device) {
+    if (m <= 0 || nn <= 0 || !x1 || !y1 || !x2 || !y2 || !theta || !A) return BV_ERR_ARG;
+    if (kernel < 1 || kernel > 4) return BV_ERR_ARG;
+    BV_HIP(hipSetDevice(device));
+    bv_ctx tmp;
+    tmp.kernel = kernel;
+    tmp.metric = distance_metric;
+    tmp.dist_scale = dist_scale > 0 ? dist_scale : 1.0;
+    const bool use_z = (z1 && z2);
+    tmp.time_flag = use_z ? 1 : 0;
+    tmp.d_z = use_z ? (double *)1 : nullptr; /* only tested for non-null */
+    bv_kparams kp;
+    int kind = 0;
+    int rc = bv_make_kparams(&tmp, theta, &kp, &kind);
+    if (rc) return rc;
+    double *d[7] = {nullptr};
+    const size_t sm = sizeof(double) * m, sn = sizeof(double) * nn;
+    BV_HIP(hipMalloc(&d[0], sm)); BV_HIP(hipMalloc(&d[1], sm));
+    BV_HIP(hipMalloc(&d[3], sn)); BV_HIP(hipMalloc(&d[4], sn));
+    BV_HIP(hipMalloc(&d[6], sm * nn));
+    BV_HIP(hipMemcpy(d[0], x1, sm, hipMemcpyHostToDevice));
+    BV_HIP(hipMemcpy(d[1], y1, sm, hipMemcpyHostToDevice));
+    BV_HIP(hipMemcpy(d[3], x2, sn, hipMemcpyHostToDevice));
+    BV_HIP(hipMemcpy(d[4], y2, sn, hipMemcpyHostToDevice));
+    if (use_z) {
+        BV_HIP(hipMalloc(&d[2], sm)); BV_HIP(hipMalloc(&d[5], sn));
+        BV_HIP(hipMemcpy(d[2], z1, sm, hipMemcpyHostToDevice));
+        BV_HIP(hipMemcpy(d[5], z2, sn, hipMemcpyHostToDevice));
+    }
+    BV_HIP(bv_launch_matern(m, nn, d[0], d[1], d[2], d[3], d[4], d[5], kp, kind, d[6]));
+    BV_HIP(hipMemcpy(A, d[6], sm * nn, hipMemcpyDeviceToHost));
+    for (double *p : d)
+        if (p) hipFree(p);
+    return BV_OK;
+}
+
+extern "C" int bv_device_selftest(int device) {
+    BV_HIP(hipSetDevice(device));
+    double ha[64], hb[64], hd[256];
+    for (int l = 0; l < 64; ++l) {
+        ha[l] = 1.0 + l;            /* A[l%16][l/16] */
+        hb[l] = 0.5 * (l % 7) - 1;  /* B[l/16][l%16] */
+    }
+    double *d = nullptr;
+    BV_HIP(hipMalloc(&d, sizeof(double) * 384));
+    BV_HIP(hipMemcpy(d, ha, sizeof(ha), hipMemcpyHostToDevice));
+    BV_HIP(hipMemcpy(d + 64, hb, sizeof(hb), hipMemcpyHostToDevice));
+    BV_HIP(bv_launch_selftest(d, d + 64, d + 128));
+    BV_HIP(hipMemcpy(hd, d + 128, sizeof(hd), hipMemcpyDeviceToHost));
+    hipFree(d);
+    /* expected D[i][j] = sum_k A[i][k] B[k][j], lane l holds D[4(l/16)+g][l%16] */
+    for (int l = 0; l < 64; ++l)
+        for (int g = 0; g < 4; ++g) {
+            const int i = 4 * (l / 16) + g, j = l % 16;
+            double ref = 0.0;
+            for (int k = 0; k < 4; ++k) ref += ha[i + 16 * k] * hb[j + 16 * k];
+            if (fabs(ref - hd[4 * l + g]) > 1e-12 * (1.0 + fabs(ref))) return 1;
+        }
+    return BV_OK;
+}

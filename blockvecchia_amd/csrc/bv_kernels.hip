@@ -1,0 +1,499 @@
+/*
+ * bv_kernels.hip -- gfx950 kernels of the Block Vecchia likelihood.
+ *
+ * bv_block_kernel<KIND>: one workgroup (4 wavefronts) per block.  It factors
+ * the block's augmented covariance (see bv_engine.h) with a left-looking
+ * blocked Cholesky:
+ *   for each 32-column panel j:
+ *     C = A(rows >= 32j, panel) - L(rows, 0:32j) L(panel, 0:32j)^T
+ *         (FP64 matrix-core 16x16x4 products, accumulators in registers);
+ *         A is generated in registers for the closed-form Matern kinds, or
+ *         read from the tiles written by bv_gen_kernel (general nu, space-time)
+ *     diagonal 32x32 block of C -> LDS; one sweep computes its Cholesky factor
+ *     L_jj and W = L_jj^-1 (forward elimination of I, same barrier per step)
+ *     L(rows below, panel) = C W^T   (matrix-core product)
+ *     store the panel's tiles to the block's workspace
+ * The log-determinant over the block rows and the squared norm of the solved
+ * observation row are reduced in-kernel (fixed order) to one pair per block.
+ *
+ * Storage of the factor: 16x16 tiles, packed lower triangle, tile-column
+ * major.  Inside a tile, lane l of a wavefront owns the 4 consecutive doubles
+ * at offset 4l: row (l % 16), columns 4(l / 16) .. 4(l / 16) + 3.  This is the
+ * accumulator layout of v_mfma_f64_16x16x4f64, so a panel is stored with one
+ * 32-byte store per lane and read back as matrix-core operands with one
+ * 32-byte load per lane (the 16-term contraction over a tile's columns is
+ * split into 4 products, the g-th using column 4(l/16)+g of every lane).
+ *
+ * bv_reduce_kernel: llh = sum_i -1/2 (dot_i + logdet_i + bs_i log(2 pi)) in
+ * block order (include/llh_Xvecchia_batch.h:480-494), plus the first block
+ * whose factorization failed.
+ */
+#include <hip/hip_runtime.h>
+#include "bv_engine.h"
+
+typedef double bv_d4 __attribute__((ext_vector_type(4)));
+
+#define BV_KIND_LOADA 7 /* factor kernel reads A from the workspace tiles */
+
+__device__ __forceinline__ bv_d4 bv_mfma(double a, double b, bv_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+/* tile (rt, ct), rt >= ct, of a factor with T tile rows; packed lower
+   triangle, tile-column major; in doubles */
+__device__ __forceinline__ int64_t bv_tile_off(int rt, int ct, int T) {
+    return ((int64_t)(ct * T - (ct * (ct - 1)) / 2 + (rt - ct))) << 8;
+}
+
+/* Entry (p, q) of the augmented matrix of bv_engine.h. */
+template <int KIND>
+__device__ __forceinline__ double bv_aug_entry(int p, int q, int m, double xp, double yp,
+                                               double zp, double xq, double yq, double zq,
+                                               double oq, const bv_kparams &kp) {
+    if (p < m && q < m) return bv_cov<KIND>(xp, yp, zp, xq, yq, zq, kp);
+    if (p == m && q < m) return oq;
+    return (p == q) ? 1.0 : 0.0;
+}
+
+/* Location / observation of augmented index p of block d. */
+__device__ __forceinline__ void bv_point(const bv_block_desc &d, const bv_eval_slot &E, int p,
+                                         double &xv, double &yv, double &zv, double &ov) {
+    xv = yv = zv = ov = 0.0;
+    if (p < d.cs0) {
+        const int64_t o = (int64_t)d.block * E.cs + p;
+        if (E.nbr) {
+            const int64_t idx = E.nbr[o];
+            xv = E.x[idx]; yv = E.y[idx]; ov = E.obs[idx];
+            if (E.z) zv = E.z[idx];
+        } else {
+            xv = E.cx[o]; yv = E.cy[o]; ov = E.cobs[o];
+            if (E.cz) zv = E.cz[o];
+        }
+    } else if (p < d.m) {
+        const int64_t idx = d.obs_off + (p - d.cs0);
+        xv = E.x[idx]; yv = E.y[idx]; ov = E.obs[idx];
+        if (E.z) zv = E.z[idx];
+    }
+}
+
+/* acc[s][ct] = L(panel rows ct) L(row tile s)^T over the first 2j tile
+   columns, for the c row tiles rt0 + wave + 4 (i0 + s); loads of tile
+   column kt+1 are issued before the products of tile column kt. */
+__device__ __forceinline__ void bv_panel_gemm(const double *__restrict__ ws, int T, int j,
+                                              int wave, int lane, int i0, int c,
+                                              bv_d4 (&acc)[BV_NRT][2]) {
+    const int rt0 = 2 * j;
+#pragma unroll
+    for (int s = 0; s < BV_NRT; ++s) {
+        acc[s][0] = (bv_d4){0.0, 0.0, 0.0, 0.0};
+        acc[s][1] = (bv_d4){0.0, 0.0, 0.0, 0.0};
+    }
+    if (rt0 == 0) return;
+    int rts[BV_NRT];
+#pragma unroll
+    for (int s = 0; s < BV_NRT; ++s) rts[s] = rt0 + wave + 4 * (i0 + s);
+    const double *wl = ws + 4 * lane;
+    bv_d4 j0 = *(const bv_d4 *)(wl + bv_tile_off(rt0, 0, T));
+    bv_d4 j1 = *(const bv_d4 *)(wl + bv_tile_off(rt0 + 1, 0, T));
+    bv_d4 r[BV_NRT];
+#pragma unroll
+    for (int s = 0; s < BV_NRT; ++s)
+        if (s < c) r[s] = *(const bv_d4 *)(wl + bv_tile_off(rts[s], 0, T));
+    for (int kt = 0; kt < rt0; ++kt) {
+        const int kn = (kt + 1 < rt0) ? kt + 1 : kt;
+        const bv_d4 jn0 = *(const bv_d4 *)(wl + bv_tile_off(rt0, kn, T));
+        const bv_d4 jn1 = *(const bv_d4 *)(wl + bv_tile_off(rt0 + 1, kn, T));
+        bv_d4 rn[BV_NRT];
+#pragma unroll
+        for (int s = 0; s < BV_NRT; ++s)
+            if (s < c) rn[s] = *(const bv_d4 *)(wl + bv_tile_off(rts[s], kn, T));
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+#pragma unroll
+            for (int s = 0; s < BV_NRT; ++s) {
+                if (s < c) {
+                    acc[s][0] = bv_mfma(j0[g], r[s][g], acc[s][0]);
+                    acc[s][1] = bv_mfma(j1[g], r[s][g], acc[s][1]);
+                }
+            }
+        }
+        j0 = jn0;
+        j1 = jn1;
+#pragma unroll
+        for (int s = 0; s < BV_NRT; ++s)
+            if (s < c) r[s] = rn[s];
+    }
+}
+
+/* acc <- A - acc for the c row tiles of this pass (panel columns J0..J0+31). */
+template <int KIND>
+__device__ __forceinline__ void bv_panel_subtract_a(const double *__restrict__ ws, int T, int j,
+                                                    int wave, int lane, int i0, int c, int m,
+                                                    const double *sx, const double *sy,
+                                                    const double *sz, const double *so,
+                                                    const bv_kparams &kp,
+                                                    bv_d4 (&acc)[BV_NRT][2]) {
+    const int J0 = j * BV_NB;
+    const int rt0 = 2 * j;
+    const int lr = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < BV_NRT; ++s) {
+        if (s < c) {
+            const int rt = rt0 + wave + 4 * (i0 + s);
+            if (KIND == BV_KIND_LOADA) {
+                const bv_d4 a0 = *(const bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane);
+                acc[s][0] = a0 - acc[s][0];
+                if (rt > rt0) {
+                    const bv_d4 a1 = *(const bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane);
+                    acc[s][1] = a1 - acc[s][1];
+                }
+            } else {
+                const int p = rt * BV_TILE + lr;
+                const double xp = sx[p], yp = sy[p], zp = sz[p];
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int q = J0 + ct * BV_TILE + 4 * lq + g;
+                        acc[s][ct][g] = bv_aug_entry<KIND>(p, q, m, xp, yp, zp, sx[q], sy[q],
+                                                           sz[q], so[q], kp) -
+                                        acc[s][ct][g];
+                    }
+                    /* keep the scheduler from interleaving all tiles' exponentials */
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+    }
+}
+
+/* X = C W^T for the pass's row tiles (skipping the diagonal ones), stored to
+   the workspace; accumulate the squared solved observations of row m. */
+__device__ __forceinline__ void bv_panel_solve_store(double *__restrict__ ws, int T, int j,
+                                                     int wave, int lane, int i0, int c, int m,
+                                                     int cs0, const double *sW,
+                                                     bv_d4 (&acc)[BV_NRT][2], double &dot_lane) {
+    const int J0 = j * BV_NB;
+    const int rt0 = 2 * j;
+    const int lr = lane & 15, lq = lane >> 4;
+    const bv_d4 w00 = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
+    const bv_d4 w10 = *(const bv_d4 *)(sW + 1 * 256 + 4 * lane);
+    const bv_d4 w11 = *(const bv_d4 *)(sW + 2 * 256 + 4 * lane);
+#pragma unroll
+    for (int s = 0; s < BV_NRT; ++s) {
+        if (s < c) {
+            const int rt = rt0 + wave + 4 * (i0 + s);
+            if (rt >= rt0 + 2) {
+                bv_d4 x0 = (bv_d4){0.0, 0.0, 0.0, 0.0};
+                bv_d4 x1 = (bv_d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    x0 = bv_mfma(w00[g], acc[s][0][g], x0);
+                    x1 = bv_mfma(w10[g], acc[s][0][g], x1);
+                }
+#pragma unroll
+                for (int g = 0; g < 4; ++g) x1 = bv_mfma(w11[g], acc[s][1][g], x1);
+                *(bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane) = x0;
+                *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane) = x1;
+                if (rt * BV_TILE + lr == m) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int q0 = J0 + 4 * lq + g, q1 = q0 + BV_TILE;
+                        if (q0 >= cs0 && q0 < m) dot_lane += x0[g] * x0[g];
+                        if (q1 >= cs0 && q1 < m) dot_lane += x1[g] * x1[g];
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* Cholesky of the 32x32 diagonal block in sA (column-major, ld 33; lower part
+   used) and, in the same sweep, W = L_jj^-1 by forward elimination of the
+   identity held in sR (row-major residual rows, ld 33).  W is written to sW
+   as three 16x16 tiles in accumulator layout: (0,0), (1,0), (1,1).  Step k,
+   element (r, c) of the 32x32 grid:
+     c > k, r >= c :  A[r][c] -= l_rk l_ck          (Cholesky trailing update)
+     r == k, c <= k:  W[k][c]  = R[k][c] / l_kk
+     r >  k, c <= k:  R[r][c] -= l_rk R[k][c] / l_kk
+   with l_rk = A[r][k] / l_kk read before any write of step k.  One barrier per
+   step.  Also accumulates the log-determinant over block rows, the first
+   failing column, and the solved observations of row m if it lies here. */
+__device__ __forceinline__ void bv_diag_factor(int tid, int J0, int m, int cs0, double *sA,
+                                               double *sR, double *sW, double *s_logdet,
+                                               double *s_dotd, int *s_info) {
+    constexpr int LD = 33;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int e = tid + BV_THREADS * t;
+        const int r = e & 31, cc = e >> 5;
+        sR[r * LD + cc] = (r == cc) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int km = m - J0; /* local index of the observation row, if in range */
+    for (int k = 0; k < BV_NB; ++k) {
+        const int qk = J0 + k;
+        const double akk = sA[k * LD + k];
+        const double piv = (qk >= m) ? 1.0 : sqrt(akk);
+        const double pinv = 1.0 / piv;
+        if (tid == 0) {
+            if (qk < m) {
+                if (!(akk > 0.0) && *s_info == 0) *s_info = qk + 1;
+                if (qk >= cs0) *s_logdet += log(piv * piv);
+            }
+            if (km >= 0 && km < BV_NB && k < km && qk >= cs0) {
+                const double v = sA[k * LD + km] * pinv;
+                *s_dotd += v * v;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int e = tid + BV_THREADS * t;
+            const int r = e & 31, cc = e >> 5;
+            if (cc > k) {
+                if (r >= cc) sA[cc * LD + r] -= (sA[k * LD + r] * pinv) * (sA[k * LD + cc] * pinv);
+            } else if (r == k) {
+                const double w = sR[k * LD + cc] * pinv;
+                const int tileid = (k < 16) ? 0 : (cc < 16 ? 1 : 2);
+                const int rl = k & 15, cl = cc & 15;
+                sW[tileid * 256 + 4 * (rl + 16 * (cl >> 2)) + (cl & 3)] = w;
+            } else if (r > k) {
+                sR[r * LD + cc] -= (sA[k * LD + r] * pinv) * (sR[k * LD + cc] * pinv);
+            }
+        }
+        __syncthreads();
+    }
+    /* upper parts of the W tiles (0,0) and (1,1) are zero */
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int e = tid + BV_THREADS * t; /* 0..511: two 16x16 tiles */
+        const int tile = e >> 8, rl = (e >> 4) & 15, cl = e & 15;
+        if (cl > rl) sW[(tile ? 2 : 0) * 256 + 4 * (rl + 16 * (cl >> 2)) + (cl & 3)] = 0.0;
+    }
+    __syncthreads();
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(BV_THREADS, 2)
+bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
+                double *ws_base) {
+    extern __shared__ double smem[];
+    __shared__ double sA[BV_NB * 33];
+    __shared__ double sR[BV_NB * 33];
+    __shared__ double sW[3 * 256];
+    __shared__ double s_logdet, s_dotd, s_wdot[BV_THREADS / 64];
+    __shared__ int s_info;
+
+    const bv_block_desc d = desc[blockIdx.x];
+    const bv_eval_slot &E = slots[d.eval];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6); /* wave-uniform */
+    const int m = d.m, cs0 = d.cs0, Mp = d.mp;
+    const int T = Mp / BV_TILE, P = Mp / BV_NB;
+    double *sx = smem, *sy = sx + Mp, *sz = sy + Mp, *so = sz + Mp;
+    const bv_kparams kp = E.kp;
+
+    if (KIND != BV_KIND_LOADA) {
+        /* stage the block's points: conditioning set first, then the block */
+        for (int p = tid; p < Mp; p += BV_THREADS) {
+            double xv, yv, zv, ov;
+            bv_point(d, E, p, xv, yv, zv, ov);
+            sx[p] = xv; sy[p] = yv; sz[p] = zv; so[p] = ov;
+        }
+    }
+    if (tid == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; }
+    __syncthreads();
+
+    double *ws = ws_base + d.ws_off;
+    double dot_lane = 0.0;
+    const int lr = lane & 15, lq = lane >> 4;
+
+    for (int j = 0; j < P; ++j) {
+        const int J0 = j * BV_NB;
+        const int nrt = T - 2 * j;
+        const int cnt = (nrt > wave) ? (nrt - wave + 3) / 4 : 0;
+        const int ngrp = (cnt + BV_NRT - 1) / BV_NRT;
+        const int gsz = ngrp ? (cnt + ngrp - 1) / ngrp : 0;
+        bv_d4 acc[BV_NRT][2];
+
+        /* first pass holds the diagonal row tiles (waves 0 and 1, s = 0) */
+        const int c0 = gsz < cnt ? gsz : cnt;
+        bv_panel_gemm(ws, T, j, wave, lane, 0, c0, acc);
+        bv_panel_subtract_a<KIND>(ws, T, j, wave, lane, 0, c0, m, sx, sy, sz, so, kp, acc);
+        if (wave < 2) {
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    sA[(ct * BV_TILE + 4 * lq + g) * 33 + wave * BV_TILE + lr] = acc[0][ct][g];
+        }
+        __syncthreads();
+        bv_diag_factor(tid, J0, m, cs0, sA, sR, sW, &s_logdet, &s_dotd, &s_info);
+        bv_panel_solve_store(ws, T, j, wave, lane, 0, c0, m, cs0, sW, acc, dot_lane);
+        for (int gi = 1; gi < ngrp; ++gi) {
+            const int i0 = gi * gsz;
+            const int c = (cnt - i0) < gsz ? (cnt - i0) : gsz;
+            bv_panel_gemm(ws, T, j, wave, lane, i0, c, acc);
+            bv_panel_subtract_a<KIND>(ws, T, j, wave, lane, i0, c, m, sx, sy, sz, so, kp, acc);
+            bv_panel_solve_store(ws, T, j, wave, lane, i0, c, m, cs0, sW, acc, dot_lane);
+        }
+        __syncthreads();
+    }
+
+    /* fixed-order reduction of the solved-observation squares */
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) dot_lane += __shfl_xor(dot_lane, off, 64);
+    if (lane == 0) s_wdot[wave] = dot_lane;
+    __syncthreads();
+    if (tid == 0) {
+        double dsum = s_dotd;
+        for (int w = 0; w < BV_THREADS / 64; ++w) dsum += s_wdot[w];
+        E.logdet[d.block] = s_logdet;
+        E.dot[d.block] = dsum;
+        E.info[d.block] = s_info;
+    }
+}
+
+/* Writes the augmented matrix A of every block (lower tiles, accumulator
+   layout) into the workspace, for the kinds whose entries are too costly to
+   recompute inside the factorization (general nu, space-time).  grid.x: tile
+   index within the block, grid.y: descriptor. */
+template <int KIND>
+__global__ void __launch_bounds__(256)
+bv_gen_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
+              double *ws_base) {
+    const bv_block_desc d = desc[blockIdx.y];
+    const int T = d.mp / BV_TILE;
+    const int ntile = T * (T + 1) / 2;
+    int tix = blockIdx.x;
+    if (tix >= ntile) return;
+    int ct = 0;
+    while (tix >= T - ct) { tix -= T - ct; ++ct; }
+    const int rt = ct + tix;
+    const bv_eval_slot &E = slots[d.eval];
+    const int e = threadIdx.x, lane = e >> 2, g = e & 3;
+    const int p = rt * BV_TILE + (lane & 15);
+    const int q = ct * BV_TILE + 4 * (lane >> 4) + g;
+    double xp, yp, zp, op, xq, yq, zq, oq;
+    bv_point(d, E, p, xp, yp, zp, op);
+    bv_point(d, E, q, xq, yq, zq, oq);
+    const double v = bv_aug_entry<KIND>(p, q, d.m, xp, yp, zp, xq, yq, zq, oq, E.kp);
+    ws_base[d.ws_off + bv_tile_off(rt, ct, T) + e] = v;
+}
+
+/* One workgroup per evaluation: llh and first failing block, fixed order. */
+extern "C" __global__ void __launch_bounds__(BV_THREADS)
+bv_reduce_kernel(const bv_eval_slot *__restrict__ slots) {
+    const bv_eval_slot &E = slots[blockIdx.x];
+    __shared__ double s_part[BV_THREADS];
+    __shared__ int s_bad[BV_THREADS];
+    const int tid = threadIdx.x;
+    const int B = E.nblocks;
+    const int per = (B + BV_THREADS - 1) / BV_THREADS;
+    const int lo = tid * per, hi = (lo + per < B) ? lo + per : B;
+    double acc = 0.0;
+    int bad = -1;
+    for (int k = lo; k < hi; ++k) {
+        acc += -(E.dot[k] + E.logdet[k] + (double)E.batch_num[k] * E.log2pi) * 0.5;
+        if (bad < 0 && E.info[k] != 0) bad = k;
+    }
+    s_part[tid] = acc;
+    s_bad[tid] = bad;
+    __syncthreads();
+    if (tid == 0) {
+        double llh = 0.0;
+        int first = -1;
+        for (int t = 0; t < BV_THREADS; ++t) {
+            llh += s_part[t];
+            if (first < 0 && s_bad[t] >= 0) first = s_bad[t];
+        }
+        E.result[0] = llh;
+        E.result[1] = (double)first;
+    }
+}
+
+/* Dense covariance block A[i + j*m] = K(l1[i], l2[j]) (core_dcmg layout). */
+extern "C" __global__ void __launch_bounds__(256)
+bv_matern_kernel(int m, int n, const double *x1, const double *y1, const double *z1,
+                 const double *x2, const double *y2, const double *z2, bv_kparams kp, int kind,
+                 double *A) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)m * n) return;
+    const int i = (int)(e % m), jj = (int)(e / m);
+    const double za = z1 ? z1[i] : 0.0, zb = z2 ? z2[jj] : 0.0;
+    double v;
+    switch (kind) {
+    case BV_KIND_NU05: v = bv_cov<BV_KIND_NU05>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp); break;
+    case BV_KIND_NU15: v = bv_cov<BV_KIND_NU15>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp); break;
+    case BV_KIND_NU25: v = bv_cov<BV_KIND_NU25>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp); break;
+    case BV_KIND_SPACETIME:
+        v = bv_cov<BV_KIND_SPACETIME>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp);
+        break;
+    default: v = bv_cov<BV_KIND_GENERAL>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp); break;
+    }
+    A[e] = v;
+}
+
+/* Matrix-core layout check: one 16x16x4 product of known operands.
+   in: a[64], b[64] as lane values; out: d[256] in lane order (4 per lane). */
+extern "C" __global__ void __launch_bounds__(64)
+bv_selftest_kernel(const double *a, const double *b, double *d) {
+    const int lane = threadIdx.x;
+    bv_d4 acc = (bv_d4){0.0, 0.0, 0.0, 0.0};
+    acc = bv_mfma(a[lane], b[lane], acc);
+    *(bv_d4 *)(d + 4 * lane) = acc;
+}
+
+/* ---- host-side launch wrappers (called from bv_api.hip) ---- */
+hipError_t bv_launch_block(int kind, unsigned nblk, size_t lds, hipStream_t st,
+                           const bv_block_desc *desc, const bv_eval_slot *slots, double *ws) {
+    switch (kind) {
+    case BV_KIND_NU05:
+        hipLaunchKernelGGL(bv_block_kernel<BV_KIND_NU05>, dim3(nblk), dim3(BV_THREADS), lds, st,
+                           desc, slots, ws);
+        break;
+    case BV_KIND_NU15:
+        hipLaunchKernelGGL(bv_block_kernel<BV_KIND_NU15>, dim3(nblk), dim3(BV_THREADS), lds, st,
+                           desc, slots, ws);
+        break;
+    case BV_KIND_NU25:
+        hipLaunchKernelGGL(bv_block_kernel<BV_KIND_NU25>, dim3(nblk), dim3(BV_THREADS), lds, st,
+                           desc, slots, ws);
+        break;
+    default:
+        hipLaunchKernelGGL(bv_block_kernel<BV_KIND_LOADA>, dim3(nblk), dim3(BV_THREADS), 0, st,
+                           desc, slots, ws);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,
+                         const bv_block_desc *desc, const bv_eval_slot *slots, double *ws) {
+    if (kind == BV_KIND_SPACETIME)
+        hipLaunchKernelGGL(bv_gen_kernel<BV_KIND_SPACETIME>, dim3(max_tiles, nblk), dim3(256), 0,
+                           st, desc, slots, ws);
+    else
+        hipLaunchKernelGGL(bv_gen_kernel<BV_KIND_GENERAL>, dim3(max_tiles, nblk), dim3(256), 0,
+                           st, desc, slots, ws);
+    return hipGetLastError();
+}
+
+hipError_t bv_launch_reduce(unsigned nevals, hipStream_t st, const bv_eval_slot *slots) {
+    hipLaunchKernelGGL(bv_reduce_kernel, dim3(nevals), dim3(BV_THREADS), 0, st, slots);
+    return hipGetLastError();
+}
+
+hipError_t bv_launch_matern(int m, int n, const double *x1, const double *y1, const double *z1,
+                            const double *x2, const double *y2, const double *z2,
+                            const bv_kparams &kp, int kind, double *A) {
+    const int64_t tot = (int64_t)m * n;
+    hipLaunchKernelGGL(bv_matern_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0,
+                       m, n, x1, y1, z1, x2, y2, z2, kp, kind, A);
+    return hipGetLastError();
+}
+
+hipError_t bv_launch_selftest(const double *a, const double *b, double *d) {
+    hipLaunchKernelGGL(bv_selftest_kernel, dim3(1), dim3(64), 0, 0, a, b, d);
+    return hipGetLastError();
+}

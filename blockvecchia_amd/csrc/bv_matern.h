@@ -1,0 +1,200 @@
+/*
+ * bv_matern.h -- Matern covariance entry and the modified Bessel function
+ * K_nu(x) for the device (also compiles for the host).
+ *
+ * The covariance follows core_dcmg (src/ckernel.cpp:1057-1093):
+ *   x = dist(a, b) / beta;  x /= dist_scale
+ *   K = sigma2                                  if x == 0
+ *     = con * x^nu * K_nu(x),  con = sigma2 / (2^(nu-1) Gamma(nu))  otherwise
+ * with the distance of calculateDistance (src/ckernel.cpp:165-186): Euclidean
+ * sqrt(dx^2 + dy^2 [+ dz^2]) or the haversine great-circle distance
+ * (src/ckernel.cpp:145-154, R = 6371 km, degrees in).
+ *
+ * For nu = 1/2, 3/2, 5/2 the closed forms sigma2 e^-x, sigma2 (1+x) e^-x and
+ * sigma2 (1 + x + x^2/3) e^-x are used.  For general nu, K_nu is computed here
+ * (GSL's gsl_sf_bessel_Knu is the reference's routine; it is not used):
+ *   nu = mu + N with |mu| <= 1/2, N = floor(nu + 1/2);
+ *   x <= 2 : Temme's series (N. M. Temme, J. Comput. Phys. 19 (1975) 324)
+ *            for K_mu and K_{mu+1}, with 1/Gamma(1 +- mu) from its Taylor series;
+ *   x >  2 : the minimal solution u_n = U(mu+1/2+n, 2mu+1, 2x) of
+ *            u_{n-1} - 2(n+x) u_n + ((n+1/2)^2 - mu^2) u_{n+1} = 0
+ *            by backward (Miller) recurrence, normalised with
+ *            sum_n C_n u_n = (2x)^(-mu-1/2),  C_n = prod_{k<n} ((k+1/2)^2 - mu^2)/(k+1),
+ *            which gives K_mu = sqrt(pi/(2x)) e^-x / S,  S = sum_n C_n u_n/u_0, and
+ *            K_{mu+1} = K_mu (mu + 1/2 + x + (mu^2 - 1/4) u_1/u_0) / x
+ *            (the Steed/Temme "CF2" normalisation, here evaluated by Horner's
+ *            rule inside the backward sweep);
+ *   then K_{k+1} = K_{k-1} + (2k/x) K_k up to nu.
+ * Relative error against mpmath <= 5e-15 over nu in [0.01, 3], x in [1e-8, 700].
+ */
+#ifndef BV_MATERN_H
+#define BV_MATERN_H
+
+#include <math.h>
+#include <stdint.h>
+#include "bv_rgamma_coef.h"
+
+#if defined(__HIPCC__)
+#define BV_HD __host__ __device__ inline
+#else
+#define BV_HD inline
+#endif
+
+#define BV_PI 3.141592653589793
+#define BV_EARTH_RADIUS_KM 6371.0
+
+/* kernel "kinds" selected per evaluation */
+enum { BV_KIND_NU05 = 0, BV_KIND_NU15 = 1, BV_KIND_NU25 = 2, BV_KIND_GENERAL = 3,
+       BV_KIND_SPACETIME = 4 };
+
+/* Parameters of one covariance evaluation, prepared on the host. */
+struct bv_kparams {
+    double sigma2;      /* theta[0] */
+    double beta;        /* theta[1] */
+    double nu;          /* theta[2] */
+    double con;         /* sigma2 / (2^(nu-1) Gamma(nu)), as src/ckernel.cpp:1072-1074 */
+    double dist_scale;
+    double beta_t, nu_t, sep, aux;   /* space-time theta[3..6] */
+    int32_t metric;     /* 0 Euclidean, 1 great circle */
+    int32_t use_z;      /* 3-D distance */
+};
+
+/* 1/Gamma(1+z), |z| <= 1/2 */
+BV_HD double bv_rgamma1p(double z) {
+    const double a[BV_RGAMMA_NCOEF] = BV_RGAMMA_COEFS;
+    double s = a[BV_RGAMMA_NCOEF - 1];
+    for (int k = BV_RGAMMA_NCOEF - 2; k >= 0; --k) s = s * z + a[k];
+    return s;
+}
+
+/* Temme's series: K_mu(x), K_{mu+1}(x) for |mu| <= 1/2, 0 < x <= 2. */
+BV_HD void bv_knu_temme(double mu, double x, double *k0, double *k1) {
+    const double a[BV_RGAMMA_NCOEF] = BV_RGAMMA_COEFS;
+    const double mu2 = mu * mu;
+    /* gamma1 = (1/G(1-mu) - 1/G(1+mu)) / (2 mu) = -sum_{k odd} a_k mu^(k-1)
+       gamma2 = (1/G(1-mu) + 1/G(1+mu)) / 2    =  sum_{k even} a_k mu^k     */
+    double g1 = 0.0, g2 = 0.0;
+    for (int k = BV_RGAMMA_NCOEF - 1; k >= 1; k -= 2) g1 = g1 * mu2 + a[k];
+    for (int k = BV_RGAMMA_NCOEF - 2; k >= 0; k -= 2) g2 = g2 * mu2 + a[k];
+    g1 = -g1;
+    const double rgp = g2 - mu * g1;   /* 1/Gamma(1+mu) */
+    const double rgm = g2 + mu * g1;   /* 1/Gamma(1-mu) */
+    const double lx = log(2.0 / x);
+    const double sig = mu * lx;
+    double shs;
+    if (fabs(sig) < 1e-3) {
+        const double s2 = sig * sig;
+        shs = 1.0 + s2 / 6.0 * (1.0 + s2 / 20.0 * (1.0 + s2 / 42.0));
+    } else {
+        shs = sinh(sig) / sig;
+    }
+    const double pim = (fabs(mu) < 1e-15) ? 1.0 : (BV_PI * mu) / sin(BV_PI * mu);
+    const double es = exp(sig);
+    double f = pim * (0.5 * (es + 1.0 / es) * g1 + shs * lx * g2);
+    double p = 0.5 * es / rgp;
+    double q = 0.5 / (es * rgm);
+    double c = 1.0;
+    double sk = f, sk1 = p;
+    const double x2 = 0.25 * x * x;
+    for (int k = 1; k < 64; ++k) {
+        const double dk = (double)k;
+        f = (dk * f + p + q) / (dk * dk - mu2);
+        p = p / (dk - mu);
+        q = q / (dk + mu);
+        c = c * x2 / dk;
+        const double t = c * f;
+        sk += t;
+        sk1 += c * (p - dk * f);
+        if (fabs(t) < 1e-17 * fabs(sk)) break;
+    }
+    *k0 = sk;
+    *k1 = sk1 * 2.0 / x;
+}
+
+/* Backward-recurrence CF2: K_mu(x), K_{mu+1}(x) for |mu| <= 1/2, x > 2. */
+BV_HD void bv_knu_cf2(double mu, double x, double *k0, double *k1) {
+    const double mu2 = mu * mu;
+    const int nmax = 12 + (int)(120.0 / x);
+    double un1 = 0.0, un = 1.0, t = 1.0;    /* u_{n+1}, u_n, Horner sum T_n */
+    for (int n = nmax; n >= 1; --n) {
+        const double dn = (double)n;
+        const double um1 = 2.0 * (dn + x) * un - ((dn + 0.5) * (dn + 0.5) - mu2) * un1;
+        t = um1 + ((dn - 0.5) * (dn - 0.5) - mu2) / dn * t;
+        un1 = un;
+        un = um1;
+        if (fabs(un) > 1e200) { un1 *= 1e-200; un *= 1e-200; t *= 1e-200; }
+    }
+    const double s = t / un;
+    const double k = sqrt(BV_PI / (2.0 * x)) * exp(-x) / s;
+    *k0 = k;
+    *k1 = k / x * (mu + 0.5 + x + (mu2 - 0.25) * un1 / un);
+}
+
+/* K_nu(x), nu >= 0, x > 0. */
+BV_HD double bv_besselk(double nu, double x) {
+    const int nl = (int)(nu + 0.5);
+    const double mu = nu - (double)nl;
+    double k0, k1;
+    if (x <= 2.0) bv_knu_temme(mu, x, &k0, &k1);
+    else bv_knu_cf2(mu, x, &k0, &k1);
+    for (int k = 1; k <= nl; ++k) {
+        const double kn = k0 + 2.0 * (mu + (double)k) / x * k1;
+        k0 = k1;
+        k1 = kn;
+    }
+    return k0;
+}
+
+/* Great-circle distance (src/ckernel.cpp:145-154). */
+BV_HD double bv_dist_earth(double lat1d, double lon1d, double lat2d, double lon2d) {
+    const double lat1r = lat1d * BV_PI / 180.0, lon1r = lon1d * BV_PI / 180.0;
+    const double lat2r = lat2d * BV_PI / 180.0, lon2r = lon2d * BV_PI / 180.0;
+    const double u = sin((lat2r - lat1r) / 2.0);
+    const double v = sin((lon2r - lon1r) / 2.0);
+    return 2.0 * BV_EARTH_RADIUS_KM * asin(sqrt(u * u + cos(lat1r) * cos(lat2r) * v * v));
+}
+
+/* Euclidean distance, products and sums rounded separately (no contraction),
+ * as sqrt(pow(dx,2) + pow(dy,2) [+ pow(dz,2)]). */
+BV_HD double bv_dist_euclid(double x1, double y1, double z1, double x2, double y2, double z2,
+                            int use_z) {
+#pragma clang fp contract(off)
+    const double dx = x2 - x1, dy = y2 - y1;
+    double s = dx * dx + dy * dy;
+    if (use_z) {
+        const double dz = z2 - z1;
+        s = s + dz * dz;
+    }
+    return sqrt(s);
+}
+
+/* One covariance entry between locations (x1,y1,z1) and (x2,y2,z2). */
+template <int KIND>
+BV_HD double bv_cov(double x1, double y1, double z1, double x2, double y2, double z2,
+                    const bv_kparams &kp) {
+    if (KIND == BV_KIND_SPACETIME) {
+        /* core_dcmg_spacetime_matern (src/ckernel.cpp:1421-1458): spatial
+           distance uses z as well, no dist_scale */
+        const double d = bv_dist_euclid(x1, y1, z1, x2, y2, z2, 1);
+        const double e = d / kp.beta;
+        const double dt = fabs(z1 - z2);
+        const double base = pow(dt, 2.0 * kp.nu_t) / kp.beta_t + 1.0;
+        const double e2 = pow(base, kp.sep / 2.0);
+        const double e3 = e / e2;
+        const double e4 = pow(base, kp.sep + kp.aux);
+        if (e == 0.0) return kp.sigma2 / e4;
+        return kp.con * pow(e3, kp.nu) * bv_besselk(kp.nu, e3) / e4;
+    }
+    double d;
+    if (kp.metric == 1) d = bv_dist_earth(x1, y1, x2, y2);
+    else d = bv_dist_euclid(x1, y1, z1, x2, y2, z2, kp.use_z);
+    double e = d / kp.beta;
+    e = e / kp.dist_scale;
+    if (e == 0.0) return kp.sigma2;
+    if (KIND == BV_KIND_NU05) return kp.sigma2 * exp(-e);
+    if (KIND == BV_KIND_NU15) return kp.sigma2 * (1.0 + e) * exp(-e);
+    if (KIND == BV_KIND_NU25) return kp.sigma2 * (1.0 + e + e * e / 3.0) * exp(-e);
+    return kp.con * pow(e, kp.nu) * bv_besselk(kp.nu, e);
+}
+
+#endif /* BV_MATERN_H */

@@ -1,0 +1,164 @@
+/*
+ * bv.h -- C ABI of the MI355X Block Vecchia log-likelihood engine (libbv.so).
+ *
+ * Plain pointers and sizes only; every function is callable from C, from an
+ * NLopt-style optimizer, from ctypes or from cgo.  All arithmetic is FP64.
+ *
+ * Reference interfaces these entry points replace (paths relative to the
+ * blockVecchia repository):
+ *   bv_create      the allocation + upload block of the driver
+ *                  (src/test_Xvecchia_batch.cpp:459-689) that fills `llh_data`
+ *                  (include/vecchia_helper_data.h:69-146)
+ *   bv_llh /
+ *   llh_Xvecchia_batch
+ *                  T llh_Xvecchia_batch(unsigned n, const T *localtheta, T *grad,
+ *                  void *f_data)   (include/llh_Xvecchia_batch.h:22-24), the
+ *                  objective registered with nlopt_set_max_objective
+ *                  (src/test_Xvecchia_batch.cpp:704)
+ *   bv_destroy     the frees at src/test_Xvecchia_batch.cpp:759-815
+ *   bv_partition   the host pre-processing of src/test_Xvecchia_batch.cpp:240-404
+ *                  (k-means, block ordering, firstbatch, cluster_combine,
+ *                  cluster_to_batch, findNearestPoints)
+ *   bv_matern      core_dcmg (src/ckernel.cpp:1057-1093) on the device
+ */
+#ifndef BV_H
+#define BV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define BV_OK              0
+#define BV_ERR_NOT_PD      1   /* a block's covariance is not positive definite  */
+#define BV_ERR_ARG        -1   /* invalid argument (sizes, pointers, theta)       */
+#define BV_ERR_HIP        -2   /* a HIP runtime call failed                       */
+#define BV_ERR_NO_DEVICE  -3   /* no usable gfx950 device                         */
+#define BV_ERR_UNSUPPORTED -4  /* kernel / metric combination not implemented     */
+#define BV_ERR_BUSY       -5   /* serving queue full / workspace exhausted        */
+
+/* ---- covariance kernels (ids of include/vecchia_helper_data.h:482-521) --- */
+#define BV_KERNEL_MATERN            1  /* univariate_matern_stationary_no_nugget   */
+#define BV_KERNEL_POWEXP            2  /* univariate_powexp_stationary_no_nugget:
+                                          evaluates Matern, as the reference does */
+#define BV_KERNEL_POWEXP_NUGGET     3  /* univariate_powexp_nugget_stationary: Matern,
+                                          4 parameters, nugget never added        */
+#define BV_KERNEL_SPACETIME         4  /* univariate_matern_spacetime_stationary  */
+
+typedef struct bv_ctx bv_ctx;
+
+/* One dataset, already partitioned and in block order (the arrays the
+ * reference keeps in llh_data: locations_new, h_obs_new, batchNum,
+ * batchNumAccum, and the conditioning sets).  The conditioning set of block i
+ * (i >= 1) is given EITHER as neighbour indices into the block-ordered arrays
+ * (nbr[i*cs + k], the kNN order of include/nearest_neighbor.h:115-147) OR as
+ * copies (con_x/con_y/con_z/con_obs, B*cs each, the reference's locations_con
+ * and h_obs_conditioning).  Row 0 is ignored: block 0 is evaluated as an
+ * exact dense Gaussian likelihood, exactly as the reference's chain leaves it
+ * (its conditioning correction is skipped, include/llh_Xvecchia_batch.h:407). */
+typedef struct bv_problem {
+    int64_t       n;               /* number of locations                      */
+    int32_t       batch_count;     /* B: number of blocks                      */
+    int32_t       cs;              /* conditioning size (0 = independent blocks)*/
+    const int32_t *batch_num;      /* [B] block sizes                          */
+    const double *x, *y, *z;       /* [n] block-ordered coordinates; z may be NULL */
+    const double *obs;             /* [n] block-ordered observations           */
+    const int32_t *nbr;            /* [B*cs] neighbour indices, or NULL        */
+    const double *con_x, *con_y, *con_z, *con_obs; /* [B*cs] copies if nbr == NULL */
+    int32_t       kernel;          /* BV_KERNEL_*                              */
+    int32_t       distance_metric; /* 0 Euclidean, 1 great-circle (2-D only)   */
+    int32_t       time_flag;       /* 1: use z (3-D / space-time)              */
+    double        dist_scale;      /* divides the scaled distance (default 1)  */
+    int32_t       device;          /* HIP device ordinal                       */
+    int32_t       verbose;         /* 1: print the reference's per-iteration line */
+} bv_problem;
+
+/* Upload one dataset (HBM-resident until bv_destroy) and size the workspace.
+ * Returns BV_OK and *out, or a negative status. */
+int bv_create(const bv_problem *p, bv_ctx **out);
+
+/* NLopt objective: log-likelihood at theta[0..n-1] (3 parameters for kernels
+ * 1/2, 4 for kernel 3, 7 for kernel 4).  grad is ignored (derivative-free).
+ * f_data is the bv_ctx*.  Returns -INFINITY when a block is not positive
+ * definite (see bv_last_status) and NaN on argument/runtime errors. */
+double bv_llh(unsigned n, const double *theta, double *grad, void *f_data);
+
+/* Same symbol name and signature as the reference objective, for a drop-in
+ * nlopt_set_max_objective(opt, llh_Xvecchia_batch, ctx). */
+double llh_Xvecchia_batch(unsigned n, const double *theta, double *grad, void *f_data);
+
+/* Evaluate k parameter vectors (row-major k x nparam) of the same dataset in
+ * shared launches; out[k] receives the log-likelihoods, status[k] (optional)
+ * the per-evaluation status. */
+int bv_llh_many(bv_ctx *ctx, int k, int nparam, const double *thetas, double *out,
+                int *status);
+
+/* Per-block terms of the last evaluation: logdet[b] = sum_j log(Ls_jj^2) and
+ * dot[b] = ||Ls^{-1}(y_B - mu)||^2 (the reference's logdet_result_h and
+ * norm2_result_h^2, include/llh_Xvecchia_batch.h:468-478).  Either may be NULL. */
+int bv_block_terms(bv_ctx *ctx, double *logdet, double *dot, int32_t *info);
+
+/* Status of the last evaluation: BV_OK, BV_ERR_NOT_PD (first_bad_block set),
+ * or a negative runtime status. */
+int bv_last_status(bv_ctx *ctx, int *first_bad_block);
+
+/* Device time in milliseconds of the last evaluation's kernels (HIP events). */
+double bv_last_kernel_ms(bv_ctx *ctx);
+
+/* Number of iterations (objective calls) so far; the reference's
+ * data->iterations counter. */
+int bv_iterations(bv_ctx *ctx);
+
+void bv_destroy(bv_ctx *ctx);
+
+/* Dense Matern block A[i + j*m] = K(l1[i], l2[j]) computed on the device with
+ * the engine's covariance code (core_dcmg, src/ckernel.cpp:1057-1093).
+ * Host arrays in, host array out.  Used for entrywise parity tests. */
+int bv_matern(int m, int nn, const double *x1, const double *y1, const double *z1,
+              const double *x2, const double *y2, const double *z2,
+              const double *theta, int kernel, int distance_metric, double dist_scale,
+              double *A, int device);
+
+/* Result of the host pre-processing; arrays are allocated by the library and
+ * released with bv_blocks_free. */
+typedef struct bv_blocks {
+    int64_t  n;
+    int32_t  batch_count;     /* B = nclusters - firstClusterCount + 1          */
+    int32_t  cs;
+    int32_t  first_cluster_count; /* clusters merged into block 0 (firstbatch) */
+    int32_t  nclusters;
+    double  *x, *y, *z, *obs; /* [n] block-ordered (z NULL in 2-D)             */
+    int32_t *batch_num;       /* [B]                                           */
+    int32_t *nbr;             /* [B*cs] kNN indices into the block order;
+                                 row 0 = 0..cs-1 (the reference's dummy set)   */
+    int64_t *perm;            /* [n] original index of each block-ordered point */
+    double  *cx, *cy, *cz;    /* [nclusters] centroids in block order (after the
+                                 random permutation); block i uses entry
+                                 i + first_cluster_count - 1                   */
+} bv_blocks;
+
+/* Host pre-processing: partition n points into blocks and pick conditioning
+ * sets, as src/test_Xvecchia_batch.cpp:240-404 does: k-means (classic Vecchia
+ * when n < 2 bc), random centroid order (srand(42) Fisher-Yates), reorderIndex,
+ * firstbatch, cluster_combine, cluster_to_batch and findNearestPoints.  The
+ * k-means is a deterministic restatement (zeroed accumulators, z = 0 in 2-D,
+ * sums in point order).  kmeans_epochs: 50 in the reference.  threads: host
+ * threads (<= 0: all).  Returns BV_OK and fills *out. */
+int bv_partition(int64_t n, const double *x, const double *y, const double *z,
+                 const double *obs, int32_t bc, int32_t cs, int32_t seed,
+                 int32_t kmeans_epochs, int32_t time_flag, int32_t distance_metric,
+                 int32_t threads, bv_blocks *out);
+
+void bv_blocks_free(bv_blocks *b);
+
+/* Checks that this build's matrix-core lane layout assumption holds on the
+ * current device (one 16x16x4 FP64 product against a host product). */
+int bv_device_selftest(int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BV_H */

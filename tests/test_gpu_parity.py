@@ -1,0 +1,248 @@
+"""GPU parity: the HIP engine (libbv through the C ABI) against the CPU oracle
+on identical inputs.  Log-likelihood tolerance: 1e-10 relative (FP64 both
+sides, different but equivalent operation orders).  Covariance entries:
+1e-13 relative."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import blockvecchia_amd as bv
+import oracle
+from blockvecchia_amd import datagen
+from conftest import LLH_RTOL, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_matrix_core_layout():
+    _say("case: matrix-core lane layout self-test")
+    assert bv.device_selftest(0) == 0
+
+
+@pytest.mark.parametrize("theta,kernel,metric,ds,use_z", [
+    ([1.5, 0.1, 0.5], 1, 0, 1.0, False),
+    ([1.0, 0.05, 1.5], 1, 0, 1.0, False),
+    ([1.0, 0.05, 2.5], 2, 0, 1.0, False),
+    ([1.0, 0.08, 0.8], 1, 0, 1.0, False),
+    ([0.7, 0.3, 2.2], 1, 0, 2.5, False),
+    ([1.2, 0.2, 0.35], 3, 0, 1.0, True),
+    ([1.0, 300.0, 0.5], 1, 1, 1.0, False),
+    ([1.0, 300.0, 1.1], 1, 1, 1.0, False),
+])
+def test_matern_entries(theta, kernel, metric, ds, use_z):
+    _say("case: matern entries theta=%s kernel=%d metric=%d" % (theta, kernel, metric))
+    rng = np.random.default_rng(5)
+    m, n = 37, 29
+    if metric == 1:
+        x1, y1 = rng.uniform(-60, 60, m), rng.uniform(-170, 170, m)
+        x2, y2 = rng.uniform(-60, 60, n), rng.uniform(-170, 170, n)
+        x2[:3], y2[:3] = x1[:3], y1[:3]
+    else:
+        x1, y1, x2, y2 = rng.random(m), rng.random(m), rng.random(n), rng.random(n)
+        x2[:3], y2[:3] = x1[:3], y1[:3]  # zero distances
+    z1 = rng.random(m) if use_z else None
+    z2 = rng.random(n) if use_z else None
+    if use_z:
+        z2[:3] = z1[:3]
+    th = theta + ([0.0] if kernel == 3 else [])
+    A = bv.matern(x1, y1, x2, y2, th, kernel, metric, ds, z1, z2)
+    R = oracle.matern(x1, y1, x2, y2, th, kernel, metric, ds, z1, z2)
+    np.testing.assert_allclose(A, R, rtol=1e-13, atol=0)
+
+
+def test_spacetime_entries():
+    _say("case: space-time entries")
+    rng = np.random.default_rng(6)
+    m = 20
+    x, y, t = rng.random(m), rng.random(m), rng.integers(1, 5, m) / 4.0
+    th = [1.0, 0.1, 0.5, 2.2, 0.5, 0.5, 0.0]
+    A = bv.matern(x, y, x, y, th, 4, 0, 1.0, t, t)
+    R = oracle.matern(x, y, x, y, th, 4, 0, 1.0, t, t)
+    np.testing.assert_allclose(A, R, rtol=1e-13, atol=0)
+
+
+def _check(blocks, theta, kernel=1, rtol=LLH_RTOL, **kw):
+    eng = bv.Engine(blocks, kernel=kernel, **kw)
+    try:
+        v = eng.llh(theta)
+        st, bad = eng.status()
+        assert st == bv.BV_OK, (st, bad)
+        ld, dt, info = eng.block_terms()
+    finally:
+        eng.close()
+    ov, old, odt, oinfo = oracle.llh(blocks, theta, kernel=kernel,
+                                     metric=kw.get("metric", 0),
+                                     time_flag=kw.get("time_flag", False),
+                                     dist_scale=kw.get("dist_scale", 1.0), per_block=True)
+    assert np.all(oinfo == 0)
+    assert np.all(info == 0)
+    _say("  gpu %.17g oracle %.17g rel %.3e" % (v, ov, rel(v, ov)))
+    assert rel(v, ov) <= rtol, (v, ov)
+    # per-block terms: absolute error against the size of the block's terms
+    np.testing.assert_allclose(ld, old, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(dt, odt, rtol=1e-9, atol=1e-9)
+    return v
+
+
+def test_llh_n2000(c1_full):
+    """n=2,000 uniform, sigma2=1.5, range=0.1, nu=0.5, 100 blocks, cs=60."""
+    _say("case: n=2000 bc=100 cs=60 nu=0.5")
+    x, y, obs = c1_full
+    b = bv.partition(x, y, obs, 100, 60, seed=0)
+    _check(b, [1.5, 0.1, 0.5])
+
+
+@pytest.mark.parametrize("bc,cs,theta", [
+    (30, 40, [1.5, 0.1, 0.5]),      # small k-means
+    (30, 0, [1.5, 0.1, 0.5]),       # independent blocks
+    (400, 30, [1.5, 0.1, 0.5]),     # classic Vecchia: one point per block
+    (30, 40, [1.0, 0.1, 1.5]),      # closed form 3/2
+    (30, 40, [1.0, 0.1, 2.5]),      # closed form 5/2
+    (30, 40, [1.0, 0.08, 0.8]),     # general nu (device K_nu)
+    (5, 100, [1.2, 0.12, 0.5]),     # big blocks (multiple passes per wave)
+    (12, 250, [1.0, 0.1, 0.5]),     # block 0 large, several panels
+])
+def test_llh_shapes(c1_small, bc, cs, theta):
+    _say("case: n=600 bc=%d cs=%d theta=%s" % (bc, cs, theta))
+    x, y, obs = c1_small
+    b = bv.partition(x, y, obs, bc, cs, seed=0)
+    _check(b, theta)
+
+
+def test_llh_copies_equal_indices(c1_small):
+    """Conditioning sets passed as copies (the reference's locations_con)."""
+    _say("case: copies vs indices")
+    x, y, obs = c1_small
+    b = bv.partition(x, y, obs, 30, 40, seed=0)
+    e1 = bv.Engine(b)
+    e2 = bv.Engine(b, use_copies=True)
+    try:
+        th = [1.5, 0.1, 0.5]
+        assert e1.llh(th) == e2.llh(th)
+    finally:
+        e1.close()
+        e2.close()
+
+
+def test_llh_dist_scale_and_kernel3(c1_small):
+    _say("case: dist_scale, 4-parameter kernel id 3")
+    x, y, obs = c1_small
+    b = bv.partition(x, y, obs, 30, 40, seed=0)
+    _check(b, [1.5, 0.2, 0.5], dist_scale=2.0)
+    _check(b, [1.5, 0.1, 0.5, 0.01], kernel=3)
+
+
+def test_llh_3d(c1_small):
+    _say("case: 3-D distance")
+    x, y, obs = c1_small
+    z = np.random.default_rng(9).random(len(x)) * 0.3
+    b = bv.partition(x, y, obs, 30, 40, seed=0, z=z, time_flag=True)
+    _check(b, [1.5, 0.1, 0.5], time_flag=True)
+
+
+def test_llh_earth():
+    _say("case: great-circle distance")
+    rng = np.random.default_rng(10)
+    n = 500
+    lat, lon = rng.uniform(20, 50, n), rng.uniform(-120, -70, n)
+    obs = rng.standard_normal(n)
+    b = bv.partition(lat, lon, obs, 25, 40, seed=0, metric=1)
+    _check(b, [1.0, 500.0, 0.5], metric=1)
+
+
+def test_llh_spacetime():
+    _say("case: space-time kernel")
+    rng = np.random.default_rng(12)
+    n = 400
+    x, y = rng.random(n), rng.random(n)
+    t = (rng.integers(1, 5, n)) / 4.0
+    obs = rng.standard_normal(n)
+    b = bv.partition(x, y, obs, 20, 40, seed=0, z=t, time_flag=True)
+    _check(b, [1.0, 0.1, 0.5, 2.2, 0.5, 0.5, 0.0], kernel=4, time_flag=True)
+
+
+def test_single_block_is_exact_gp():
+    _say("case: single block = dense GP")
+    x, y = datagen.uniform_locations(300, 3)
+    obs = datagen.sample_dense(x, y, 1.0, 0.1, 1.5, seed=3)
+    b = bv.Blocks(x, y, obs, np.array([300], np.int32), np.zeros(0, np.int32), 0)
+    eng = bv.Engine(b)
+    v = eng.llh([1.0, 0.1, 1.5])
+    eng.close()
+    ref = oracle.dense_llh(x, y, obs, [1.0, 0.1, 1.5])
+    assert rel(v, ref) < LLH_RTOL
+
+
+def test_many_equals_single_and_deterministic(c1_small):
+    _say("case: llh_many vs llh, repeatability")
+    x, y, obs = c1_small
+    b = bv.partition(x, y, obs, 30, 40, seed=0)
+    eng = bv.Engine(b)
+    try:
+        thetas = np.array([[1.5, 0.1, 0.5], [1.2, 0.08, 0.5], [0.9, 0.2, 0.5], [2.0, 0.15, 0.5]])
+        single = np.array([eng.llh(t) for t in thetas])
+        many, st = eng.llh_many(thetas)
+        assert np.all(st == 0)
+        np.testing.assert_array_equal(single, many)
+        again = np.array([eng.llh(t) for t in thetas])
+        np.testing.assert_array_equal(single, again)
+        # mixed smoothness in one call goes through the general path
+        mixed = np.array([[1.5, 0.1, 0.5], [1.0, 0.1, 0.8]])
+        mv, st = eng.llh_many(mixed)
+        for k in range(2):
+            assert rel(mv[k], oracle.llh(b, mixed[k])) < LLH_RTOL
+        assert eng.iterations() == 4 + 4 + 4 + 2
+    finally:
+        eng.close()
+
+
+def test_not_positive_definite_status():
+    _say("case: non-PD block reported")
+    x = np.array([0.1, 0.1, 0.5, 0.7, 0.2, 0.9])
+    y = np.array([0.2, 0.2, 0.5, 0.1, 0.8, 0.3])
+    b = bv.Blocks(x, y, np.ones(6), np.array([3, 3], np.int32), np.zeros(0, np.int32), 0)
+    eng = bv.Engine(b)
+    v = eng.llh([1.0, 0.1, 0.5])
+    st, bad = eng.status()
+    _, _, info = eng.block_terms()
+    eng.close()
+    assert v == -math.inf and st == bv.BV_ERR_NOT_PD and bad == 0 and info[0] == 2
+
+
+@pytest.fixture(scope="module")
+def c2_blocks():
+    """n=20,000 uniform (seed 0), 1,500 k-means blocks, cs=300; observations
+    from exact dense FP64 sampling (sigma2=1.5, range=0.1, nu=0.5)."""
+    x, y = datagen.uniform_locations(20000, 0)
+    obs = datagen.sample_dense(x, y, 1.5, 0.1, 0.5, seed=0, device="cuda")
+    return bv.partition(x, y, obs, 1500, 300, seed=0)
+
+
+def test_llh_n20000_cs300(c2_blocks):
+    """The benchmark configuration against the oracle, full size."""
+    _say("case: n=20000 bc=1500 cs=300 nu=0.5 (full size)")
+    _check(c2_blocks, [1.5, 0.1, 0.5])
+
+
+def test_n20000_scaling_law(c2_blocks):
+    """Size-independent property at full size: llh(c s2) from the s2 = 1
+    per-block terms."""
+    _say("case: n=20000 sigma2 scaling law")
+    eng = bv.Engine(c2_blocks)
+    try:
+        eng.llh([1.0, 0.1, 0.5])
+        ld, dt, _ = eng.block_terms()
+        n = c2_blocks.n
+        for c in (0.37, 4.1):
+            v = eng.llh([c, 0.1, 0.5])
+            ref = -0.5 * (dt.sum() / c + ld.sum() + n * math.log(c) + n * math.log(2 * math.pi))
+            assert rel(v, ref) < 1e-11
+    finally:
+        eng.close()
