@@ -471,10 +471,10 @@ extern "C" int bv_device_selftest(int device) {
     BV_HIP(bv_launch_selftest(d, d + 64, d + 128));
     BV_HIP(hipMemcpy(hd, d + 128, sizeof(hd), hipMemcpyDeviceToHost));
     hipFree(d);
-    /* expected D[i][j] = sum_k A[i][k] B[k][j], lane l holds D[4(l/16)+g][l%16] */
+    /* expected D[i][j] = sum_k A[i][k] B[k][j], lane l holds D[4g + l/16][l%16] */
     for (int l = 0; l < 64; ++l)
         for (int g = 0; g < 4; ++g) {
-            const int i = 4 * (l / 16) + g, j = l % 16;
+            const int i = 4 * g + l / 16, j = l % 16;
             double ref = 0.0;
             for (int k = 0; k < 4; ++k) ref += ha[i + 16 * k] * hb[j + 16 * k];
             if (fabs(ref - hd[4 * l + g]) > 1e-12 * (1.0 + fabs(ref))) return 1;

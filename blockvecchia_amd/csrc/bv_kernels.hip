@@ -18,11 +18,13 @@
  *
  * Storage of the factor: 16x16 tiles, packed lower triangle, tile-column
  * major.  Inside a tile, lane l of a wavefront owns the 4 consecutive doubles
- * at offset 4l: row (l % 16), columns 4(l / 16) .. 4(l / 16) + 3.  This is the
- * accumulator layout of v_mfma_f64_16x16x4f64, so a panel is stored with one
- * 32-byte store per lane and read back as matrix-core operands with one
- * 32-byte load per lane (the 16-term contraction over a tile's columns is
- * split into 4 products, the g-th using column 4(l/16)+g of every lane).
+ * at offset 4l: row (l % 16), columns (l / 16) + 4g for g = 0..3.  This is the
+ * accumulator layout of v_mfma_f64_16x16x4f64 for the transposed product
+ * (its D[4g + l/16][l % 16] sits in lane l, register g), so a panel is stored
+ * with one 32-byte store per lane and read back as matrix-core operands with
+ * one 32-byte load per lane: the 16-term contraction over a tile's columns is
+ * split into 4 products, the g-th using column 4g + l/16 of every lane, which
+ * is where both operands hold it.
  *
  * bv_reduce_kernel: llh = sum_i -1/2 (dot_i + logdet_i + bs_i log(2 pi)) in
  * block order (include/llh_Xvecchia_batch.h:480-494), plus the first block
@@ -154,7 +156,7 @@ __device__ __forceinline__ void bv_panel_subtract_a(const double *__restrict__ w
                 for (int ct = 0; ct < 2; ++ct) {
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
-                        const int q = J0 + ct * BV_TILE + 4 * lq + g;
+                        const int q = J0 + ct * BV_TILE + 4 * g + lq;
                         acc[s][ct][g] = bv_aug_entry<KIND>(p, q, m, xp, yp, zp, sx[q], sy[q],
                                                            sz[q], so[q], kp) -
                                         acc[s][ct][g];
@@ -198,7 +200,7 @@ __device__ __forceinline__ void bv_panel_solve_store(double *__restrict__ ws, in
                 if (rt * BV_TILE + lr == m) {
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
-                        const int q0 = J0 + 4 * lq + g, q1 = q0 + BV_TILE;
+                        const int q0 = J0 + 4 * g + lq, q1 = q0 + BV_TILE;
                         if (q0 >= cs0 && q0 < m) dot_lane += x0[g] * x0[g];
                         if (q1 >= cs0 && q1 < m) dot_lane += x1[g] * x1[g];
                     }
@@ -256,7 +258,7 @@ __device__ __forceinline__ void bv_diag_factor(int tid, int J0, int m, int cs0, 
                 const double w = sR[k * LD + cc] * pinv;
                 const int tileid = (k < 16) ? 0 : (cc < 16 ? 1 : 2);
                 const int rl = k & 15, cl = cc & 15;
-                sW[tileid * 256 + 4 * (rl + 16 * (cl >> 2)) + (cl & 3)] = w;
+                sW[tileid * 256 + 4 * (rl + 16 * (cl & 3)) + (cl >> 2)] = w;
             } else if (r > k) {
                 sR[r * LD + cc] -= (sA[k * LD + r] * pinv) * (sR[k * LD + cc] * pinv);
             }
@@ -268,7 +270,7 @@ __device__ __forceinline__ void bv_diag_factor(int tid, int J0, int m, int cs0, 
     for (int t = 0; t < 2; ++t) {
         const int e = tid + BV_THREADS * t; /* 0..511: two 16x16 tiles */
         const int tile = e >> 8, rl = (e >> 4) & 15, cl = e & 15;
-        if (cl > rl) sW[(tile ? 2 : 0) * 256 + 4 * (rl + 16 * (cl >> 2)) + (cl & 3)] = 0.0;
+        if (cl > rl) sW[(tile ? 2 : 0) * 256 + 4 * (rl + 16 * (cl & 3)) + (cl >> 2)] = 0.0;
     }
     __syncthreads();
 }
@@ -325,7 +327,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
             for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
                 for (int g = 0; g < 4; ++g)
-                    sA[(ct * BV_TILE + 4 * lq + g) * 33 + wave * BV_TILE + lr] = acc[0][ct][g];
+                    sA[(ct * BV_TILE + 4 * g + lq) * 33 + wave * BV_TILE + lr] = acc[0][ct][g];
         }
         __syncthreads();
         bv_diag_factor(tid, J0, m, cs0, sA, sR, sW, &s_logdet, &s_dotd, &s_info);
@@ -373,7 +375,7 @@ bv_gen_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__rest
     const bv_eval_slot &E = slots[d.eval];
     const int e = threadIdx.x, lane = e >> 2, g = e & 3;
     const int p = rt * BV_TILE + (lane & 15);
-    const int q = ct * BV_TILE + 4 * (lane >> 4) + g;
+    const int q = ct * BV_TILE + 4 * g + (lane >> 4);
     double xp, yp, zp, op, xq, yq, zq, oq;
     bv_point(d, E, p, xp, yp, zp, op);
     bv_point(d, E, q, xq, yq, zq, oq);
