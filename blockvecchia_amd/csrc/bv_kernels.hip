@@ -5,14 +5,15 @@
  * the block's augmented covariance (see bv_engine.h) with a left-looking
  * blocked Cholesky:
  *   for each 32-column panel j:
- *     C = A(rows >= 32j, panel) - L(rows, 0:32j) L(panel, 0:32j)^T
- *         (FP64 matrix-core 16x16x4 products, accumulators in registers);
- *         A is generated in registers for the closed-form Matern kinds, or
- *         read from the tiles written by bv_gen_kernel (general nu, space-time)
- *     diagonal 32x32 block of C -> LDS; one sweep computes its Cholesky factor
- *     L_jj and W = L_jj^-1 (forward elimination of I, same barrier per step)
- *     L(rows below, panel) = C W^T   (matrix-core product)
- *     store the panel's tiles to the block's workspace
+ *     0: A(rows >= 32j, panel) generated into the panel's workspace tiles
+ *        (closed-form Matern kinds; general nu and space-time entries are
+ *        written beforehand by bv_gen_kernel)
+ *     1: C = A - L(rows, 0:32j) L(panel, 0:32j)^T (FP64 matrix-core 16x16x4
+ *        products, accumulators in registers), stored over A
+ *     2: diagonal 32x32 block of C -> LDS; one sweep computes its Cholesky
+ *        factor L_jj and W = L_jj^-1 (forward elimination of I, one barrier
+ *        per step)
+ *     3: L(rows below, panel) = C W^T (matrix-core products), stored over C
  * The log-determinant over the block rows and the squared norm of the solved
  * observation row are reduced in-kernel (fixed order) to one pair per block.
  *
@@ -78,134 +79,158 @@ __device__ __forceinline__ void bv_point(const bv_block_desc &d, const bv_eval_s
     }
 }
 
-/* acc[s][ct] = L(panel rows ct) L(row tile s)^T over the first 2j tile
-   columns, for the c row tiles rt0 + wave + 4 (i0 + s); loads of tile
-   column kt+1 are issued before the products of tile column kt. */
+/* acc[s][ct] -= L(panel rows ct) L(row tile s)^T over the first 2j tile
+   columns, for the C row tiles rt0 + wave + 4 (i0 + s) (acc holds A on entry;
+   the panel operand is negated, so the products subtract).  C is a compile-time
+   count so the loop is straight-line matrix-core code; the loads of tile
+   column kt+1 are in flight while the products of tile column kt issue. */
+template <int C>
+__device__ __forceinline__ void bv_gemm_step(const bv_d4 &j0, const bv_d4 &j1, const bv_d4 (&r)[C],
+                                             bv_d4 (&acc)[BV_NRT][2]) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+            acc[s][0] = bv_mfma(j0[g], r[s][g], acc[s][0]);
+            acc[s][1] = bv_mfma(j1[g], r[s][g], acc[s][1]);
+        }
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void bv_gemm_load(const double *colp, int rt0, int wave, int i0, int lo,
+                                             bv_d4 &j0, bv_d4 &j1, bv_d4 (&r)[C]) {
+    j0 = *(const bv_d4 *)(colp + ((int64_t)rt0 << 8) + lo);
+    j1 = *(const bv_d4 *)(colp + ((int64_t)(rt0 + 1) << 8) + lo);
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+        r[s] = *(const bv_d4 *)(colp + ((int64_t)(rt0 + wave + 4 * (i0 + s)) << 8) + lo);
+}
+
+/* acc += L(panel rows ct) L(row tile s)^T over the first 2j tile columns, for
+   the C row tiles rt0 + wave + 4 (i0 + s).  C is a compile-time count so the
+   loop is straight-line matrix-core code.  Two operand buffers alternate: the
+   loads of tile column kt+1 are in flight while the products of column kt
+   issue, with no register copies between iterations.  Tile (rt, kt) sits at
+   ws + (kt T - kt(kt-1)/2 + rt - kt) * 256: the column part is wave-uniform and
+   advances by T - kt - 1 tiles per column, the lane part is 4 * lane. */
+template <int C>
 __device__ __forceinline__ void bv_panel_gemm(const double *__restrict__ ws, int T, int j,
-                                              int wave, int lane, int i0, int c,
+                                              int wave, int lane, int i0,
                                               bv_d4 (&acc)[BV_NRT][2]) {
     const int rt0 = 2 * j;
-#pragma unroll
-    for (int s = 0; s < BV_NRT; ++s) {
-        acc[s][0] = (bv_d4){0.0, 0.0, 0.0, 0.0};
-        acc[s][1] = (bv_d4){0.0, 0.0, 0.0, 0.0};
-    }
     if (rt0 == 0) return;
-    int rts[BV_NRT];
+    const int lo = 4 * lane;
+    const double *colp = ws;
+    bv_d4 ja0, ja1, ra[C], jb0, jb1, rb[C];
+    bv_gemm_load<C>(colp, rt0, wave, i0, lo, ja0, ja1, ra);
+    for (int kt = 0; kt < rt0; kt += 2) {
+        /* column kt+1 (rt0 is even, so it always exists here) */
+        const double *colb = colp + ((int64_t)(T - kt - 1) << 8);
+        bv_gemm_load<C>(colb, rt0, wave, i0, lo, jb0, jb1, rb);
+        bv_gemm_step<C>(ja0, ja1, ra, acc);
+        /* column kt+2, or column kt+1 again after the last pair */
+        const double *cola = (kt + 2 < rt0) ? colb + ((int64_t)(T - kt - 2) << 8) : colb;
+        bv_gemm_load<C>(cola, rt0, wave, i0, lo, ja0, ja1, ra);
+        bv_gemm_step<C>(jb0, jb1, rb, acc);
+        colp = cola;
+    }
+}
+
+/* Phase 0 of a panel: A(rt, panel) for this wave's row tiles rt0 + wave + 4i,
+   generated from the staged locations and written to the tiles (rt, rt0),
+   (rt, rt0+1) in the lane layout phase 1 reads them back in (each lane reads
+   only what it wrote, so no barrier is needed between the two phases). */
+template <int KIND>
+__device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T, int j, int wave,
+                                                  int lane, int m, const double *sx,
+                                                  const double *sy, const double *sz,
+                                                  const double *so, const bv_kparams &kp) {
+    const int J0 = j * BV_NB;
+    const int rt0 = 2 * j;
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int rt = rt0 + wave; rt < T; rt += 4) {
+        const int p = rt * BV_TILE + lr;
+        const double xp = sx[p], yp = sy[p], zp = sz[p];
+        for (int ct = 0; ct < 2; ++ct) {
+            if (rt == rt0 && ct == 1) continue; /* upper tile of the diagonal block */
+            bv_d4 v;
 #pragma unroll
-    for (int s = 0; s < BV_NRT; ++s) rts[s] = rt0 + wave + 4 * (i0 + s);
-    const double *wl = ws + 4 * lane;
-    bv_d4 j0 = *(const bv_d4 *)(wl + bv_tile_off(rt0, 0, T));
-    bv_d4 j1 = *(const bv_d4 *)(wl + bv_tile_off(rt0 + 1, 0, T));
-    bv_d4 r[BV_NRT];
+            for (int g = 0; g < 4; ++g) {
+                const int q = J0 + ct * BV_TILE + 4 * g + lq;
+                v[g] = bv_aug_entry<KIND>(p, q, m, xp, yp, zp, sx[q], sy[q], sz[q], so[q], kp);
+            }
+            *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + ct, T) + 4 * lane) = v;
+        }
+    }
+}
+
+/* Phase 1 of a panel for C row tiles: acc = A - products, stored as C into
+   the tiles (rt, rt0) and (rt, rt0+1) (the slots L will take). */
+template <int C>
+__device__ __forceinline__ void bv_pass_compute_c(double *__restrict__ ws, int T, int j, int wave,
+                                                  int lane, int i0) {
+    bv_d4 acc[BV_NRT][2];
+    const int rt0 = 2 * j;
+    /* acc = -A; the products are added; C = -acc (negations are exact) */
 #pragma unroll
-    for (int s = 0; s < BV_NRT; ++s)
-        if (s < c) r[s] = *(const bv_d4 *)(wl + bv_tile_off(rts[s], 0, T));
-    for (int kt = 0; kt < rt0; ++kt) {
-        const int kn = (kt + 1 < rt0) ? kt + 1 : kt;
-        const bv_d4 jn0 = *(const bv_d4 *)(wl + bv_tile_off(rt0, kn, T));
-        const bv_d4 jn1 = *(const bv_d4 *)(wl + bv_tile_off(rt0 + 1, kn, T));
-        bv_d4 rn[BV_NRT];
+    for (int s = 0; s < C; ++s) {
+        const int rt = rt0 + wave + 4 * (i0 + s);
+        acc[s][0] = -*(const bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane);
+        acc[s][1] = (rt > rt0) ? -*(const bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane)
+                               : (bv_d4){0.0, 0.0, 0.0, 0.0};
+    }
+    bv_panel_gemm<C>(ws, T, j, wave, lane, i0, acc);
 #pragma unroll
-        for (int s = 0; s < BV_NRT; ++s)
-            if (s < c) rn[s] = *(const bv_d4 *)(wl + bv_tile_off(rts[s], kn, T));
+    for (int s = 0; s < C; ++s) {
+        const int rt = rt0 + wave + 4 * (i0 + s);
+        *(bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane) = -acc[s][0];
+        if (rt > rt0) *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane) = -acc[s][1];
+    }
+}
+
+__device__ __forceinline__ void bv_pass_compute(int c, double *__restrict__ ws, int T, int j,
+                                                int wave, int lane, int i0) {
+    switch (c) {
+    case 1: bv_pass_compute_c<1>(ws, T, j, wave, lane, i0); break;
+    case 2: bv_pass_compute_c<2>(ws, T, j, wave, lane, i0); break;
+    case 3: bv_pass_compute_c<3>(ws, T, j, wave, lane, i0); break;
+    case 4: bv_pass_compute_c<4>(ws, T, j, wave, lane, i0); break;
+    default: break;
+    }
+}
+
+/* Phase 3 for one row tile below the diagonal block: X = C W^T (matrix-core
+   products with the three W tiles), stored over C; squares of the solved
+   observation row m accumulated into dot_lane. */
+__device__ __forceinline__ void bv_tile_solve(double *__restrict__ ws, int T, int j, int rt,
+                                              int lane, int m, int cs0, const bv_d4 &w00,
+                                              const bv_d4 &w10, const bv_d4 &w11,
+                                              double &dot_lane) {
+    const int J0 = j * BV_NB;
+    const int rt0 = 2 * j;
+    double *t0 = ws + bv_tile_off(rt, rt0, T) + 4 * lane;
+    double *t1 = ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane;
+    const bv_d4 c0 = *(const bv_d4 *)t0;
+    const bv_d4 c1 = *(const bv_d4 *)t1;
+    bv_d4 x0 = (bv_d4){0.0, 0.0, 0.0, 0.0};
+    bv_d4 x1 = (bv_d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        x0 = bv_mfma(w00[g], c0[g], x0);
+        x1 = bv_mfma(w10[g], c0[g], x1);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) x1 = bv_mfma(w11[g], c1[g], x1);
+    *(bv_d4 *)t0 = x0;
+    *(bv_d4 *)t1 = x1;
+    const int lr = lane & 15, lq = lane >> 4;
+    if (rt * BV_TILE + lr == m) {
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-#pragma unroll
-            for (int s = 0; s < BV_NRT; ++s) {
-                if (s < c) {
-                    acc[s][0] = bv_mfma(j0[g], r[s][g], acc[s][0]);
-                    acc[s][1] = bv_mfma(j1[g], r[s][g], acc[s][1]);
-                }
-            }
-        }
-        j0 = jn0;
-        j1 = jn1;
-#pragma unroll
-        for (int s = 0; s < BV_NRT; ++s)
-            if (s < c) r[s] = rn[s];
-    }
-}
-
-/* acc <- A - acc for the c row tiles of this pass (panel columns J0..J0+31). */
-template <int KIND>
-__device__ __forceinline__ void bv_panel_subtract_a(const double *__restrict__ ws, int T, int j,
-                                                    int wave, int lane, int i0, int c, int m,
-                                                    const double *sx, const double *sy,
-                                                    const double *sz, const double *so,
-                                                    const bv_kparams &kp,
-                                                    bv_d4 (&acc)[BV_NRT][2]) {
-    const int J0 = j * BV_NB;
-    const int rt0 = 2 * j;
-    const int lr = lane & 15, lq = lane >> 4;
-#pragma unroll
-    for (int s = 0; s < BV_NRT; ++s) {
-        if (s < c) {
-            const int rt = rt0 + wave + 4 * (i0 + s);
-            if (KIND == BV_KIND_LOADA) {
-                const bv_d4 a0 = *(const bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane);
-                acc[s][0] = a0 - acc[s][0];
-                if (rt > rt0) {
-                    const bv_d4 a1 = *(const bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane);
-                    acc[s][1] = a1 - acc[s][1];
-                }
-            } else {
-                const int p = rt * BV_TILE + lr;
-                const double xp = sx[p], yp = sy[p], zp = sz[p];
-#pragma unroll
-                for (int ct = 0; ct < 2; ++ct) {
-#pragma unroll
-                    for (int g = 0; g < 4; ++g) {
-                        const int q = J0 + ct * BV_TILE + 4 * g + lq;
-                        acc[s][ct][g] = bv_aug_entry<KIND>(p, q, m, xp, yp, zp, sx[q], sy[q],
-                                                           sz[q], so[q], kp) -
-                                        acc[s][ct][g];
-                    }
-                    /* keep the scheduler from interleaving all tiles' exponentials */
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-            }
-        }
-    }
-}
-
-/* X = C W^T for the pass's row tiles (skipping the diagonal ones), stored to
-   the workspace; accumulate the squared solved observations of row m. */
-__device__ __forceinline__ void bv_panel_solve_store(double *__restrict__ ws, int T, int j,
-                                                     int wave, int lane, int i0, int c, int m,
-                                                     int cs0, const double *sW,
-                                                     bv_d4 (&acc)[BV_NRT][2], double &dot_lane) {
-    const int J0 = j * BV_NB;
-    const int rt0 = 2 * j;
-    const int lr = lane & 15, lq = lane >> 4;
-    const bv_d4 w00 = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
-    const bv_d4 w10 = *(const bv_d4 *)(sW + 1 * 256 + 4 * lane);
-    const bv_d4 w11 = *(const bv_d4 *)(sW + 2 * 256 + 4 * lane);
-#pragma unroll
-    for (int s = 0; s < BV_NRT; ++s) {
-        if (s < c) {
-            const int rt = rt0 + wave + 4 * (i0 + s);
-            if (rt >= rt0 + 2) {
-                bv_d4 x0 = (bv_d4){0.0, 0.0, 0.0, 0.0};
-                bv_d4 x1 = (bv_d4){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    x0 = bv_mfma(w00[g], acc[s][0][g], x0);
-                    x1 = bv_mfma(w10[g], acc[s][0][g], x1);
-                }
-#pragma unroll
-                for (int g = 0; g < 4; ++g) x1 = bv_mfma(w11[g], acc[s][1][g], x1);
-                *(bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane) = x0;
-                *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane) = x1;
-                if (rt * BV_TILE + lr == m) {
-#pragma unroll
-                    for (int g = 0; g < 4; ++g) {
-                        const int q0 = J0 + 4 * g + lq, q1 = q0 + BV_TILE;
-                        if (q0 >= cs0 && q0 < m) dot_lane += x0[g] * x0[g];
-                        if (q1 >= cs0 && q1 < m) dot_lane += x1[g] * x1[g];
-                    }
-                }
-            }
+            const int q0 = J0 + 4 * g + lq, q1 = q0 + BV_TILE;
+            if (q0 >= cs0 && q0 < m) dot_lane += x0[g] * x0[g];
+            if (q1 >= cs0 && q1 < m) dot_lane += x1[g] * x1[g];
         }
     }
 }
@@ -288,8 +313,8 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
 
     const bv_block_desc d = desc[blockIdx.x];
     const bv_eval_slot &E = slots[d.eval];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6); /* wave-uniform */
+    const int tid0 = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid0 >> 6); /* wave-uniform */
     const int m = d.m, cs0 = d.cs0, Mp = d.mp;
     const int T = Mp / BV_TILE, P = Mp / BV_NB;
     double *sx = smem, *sy = sx + Mp, *sz = sy + Mp, *so = sz + Mp;
@@ -297,47 +322,60 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
 
     if (KIND != BV_KIND_LOADA) {
         /* stage the block's points: conditioning set first, then the block */
-        for (int p = tid; p < Mp; p += BV_THREADS) {
+        for (int p = tid0; p < Mp; p += BV_THREADS) {
             double xv, yv, zv, ov;
             bv_point(d, E, p, xv, yv, zv, ov);
             sx[p] = xv; sy[p] = yv; sz[p] = zv; so[p] = ov;
         }
     }
-    if (tid == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; }
+    if (tid0 == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; }
     __syncthreads();
 
     double *ws = ws_base + d.ws_off;
     double dot_lane = 0.0;
-    const int lr = lane & 15, lq = lane >> 4;
 
     for (int j = 0; j < P; ++j) {
+        /* thread indices re-derived per panel: an empty asm hides that they are
+           loop-invariant, so what depends on them is recomputed here instead
+           of being kept in registers across the whole loop */
+        int tid = tid0;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & 63;
         const int J0 = j * BV_NB;
-        const int nrt = T - 2 * j;
+        const int rt0 = 2 * j;
+        const int nrt = T - rt0;
+        /* phase 1: row tiles rt0 + wave + 4i, in passes of at most BV_NRT */
         const int cnt = (nrt > wave) ? (nrt - wave + 3) / 4 : 0;
         const int ngrp = (cnt + BV_NRT - 1) / BV_NRT;
         const int gsz = ngrp ? (cnt + ngrp - 1) / ngrp : 0;
-        bv_d4 acc[BV_NRT][2];
-
-        /* first pass holds the diagonal row tiles (waves 0 and 1, s = 0) */
-        const int c0 = gsz < cnt ? gsz : cnt;
-        bv_panel_gemm(ws, T, j, wave, lane, 0, c0, acc);
-        bv_panel_subtract_a<KIND>(ws, T, j, wave, lane, 0, c0, m, sx, sy, sz, so, kp, acc);
-        if (wave < 2) {
+        if (KIND != BV_KIND_LOADA)
+            bv_panel_generate<KIND>(ws, T, j, wave, lane, m, sx, sy, sz, so, kp);
+        for (int gi = 0; gi < ngrp; ++gi) {
+            const int i0 = gi * gsz;
+            const int c = (cnt - i0) < gsz ? (cnt - i0) : gsz;
+            bv_pass_compute(c, ws, T, j, wave, lane, i0);
+        }
+        __syncthreads();
+        /* phase 2: the diagonal block's C tiles -> sA, factor and invert */
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-                for (int g = 0; g < 4; ++g)
-                    sA[(ct * BV_TILE + 4 * g + lq) * 33 + wave * BV_TILE + lr] = acc[0][ct][g];
+        for (int t = 0; t < 3; ++t) {
+            const int e = tid + BV_THREADS * t;       /* 0..767: tiles (0,0),(1,0),(1,1) */
+            const int tl = e >> 8, off = e & 255;
+            const int l = off >> 2, g = off & 3;
+            const int r = (l & 15) + (tl ? 16 : 0);
+            const int cc = 4 * g + (l >> 4) + (tl == 2 ? 16 : 0);
+            const int trt = rt0 + (tl ? 1 : 0), tct = rt0 + (tl == 2 ? 1 : 0);
+            sA[cc * 33 + r] = ws[bv_tile_off(trt, tct, T) + off];
         }
         __syncthreads();
         bv_diag_factor(tid, J0, m, cs0, sA, sR, sW, &s_logdet, &s_dotd, &s_info);
-        bv_panel_solve_store(ws, T, j, wave, lane, 0, c0, m, cs0, sW, acc, dot_lane);
-        for (int gi = 1; gi < ngrp; ++gi) {
-            const int i0 = gi * gsz;
-            const int c = (cnt - i0) < gsz ? (cnt - i0) : gsz;
-            bv_panel_gemm(ws, T, j, wave, lane, i0, c, acc);
-            bv_panel_subtract_a<KIND>(ws, T, j, wave, lane, i0, c, m, sx, sy, sz, so, kp, acc);
-            bv_panel_solve_store(ws, T, j, wave, lane, i0, c, m, cs0, sW, acc, dot_lane);
+        /* phase 3: rows below the diagonal block */
+        {
+            const bv_d4 w00 = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
+            const bv_d4 w10 = *(const bv_d4 *)(sW + 1 * 256 + 4 * lane);
+            const bv_d4 w11 = *(const bv_d4 *)(sW + 2 * 256 + 4 * lane);
+            for (int rt = rt0 + 2 + wave; rt < T; rt += 4)
+                bv_tile_solve(ws, T, j, rt, lane, m, cs0, w00, w10, w11, dot_lane);
         }
         __syncthreads();
     }
@@ -345,9 +383,9 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
     /* fixed-order reduction of the solved-observation squares */
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) dot_lane += __shfl_xor(dot_lane, off, 64);
-    if (lane == 0) s_wdot[wave] = dot_lane;
+    if ((tid0 & 63) == 0) s_wdot[wave] = dot_lane;
     __syncthreads();
-    if (tid == 0) {
+    if (tid0 == 0) {
         double dsum = s_dotd;
         for (int w = 0; w < BV_THREADS / 64; ++w) dsum += s_wdot[w];
         E.logdet[d.block] = s_logdet;
