@@ -235,69 +235,137 @@ __device__ __forceinline__ void bv_tile_solve(double *__restrict__ ws, int T, in
     }
 }
 
-/* Cholesky of the 32x32 diagonal block in sA (column-major, ld 33; lower part
-   used) and, in the same sweep, W = L_jj^-1 by forward elimination of the
-   identity held in sR (row-major residual rows, ld 33).  W is written to sW
-   as three 16x16 tiles in accumulator layout: (0,0), (1,0), (1,1).  Step k,
-   element (r, c) of the 32x32 grid:
-     c > k, r >= c :  A[r][c] -= l_rk l_ck          (Cholesky trailing update)
-     r == k, c <= k:  W[k][c]  = R[k][c] / l_kk
-     r >  k, c <= k:  R[r][c] -= l_rk R[k][c] / l_kk
-   with l_rk = A[r][k] / l_kk read before any write of step k.  One barrier per
-   step.  Also accumulates the log-determinant over block rows, the first
-   failing column, and the solved observations of row m if it lies here. */
-__device__ __forceinline__ void bv_diag_factor(int tid, int J0, int m, int cs0, double *sA,
-                                               double *sR, double *sW, double *s_logdet,
-                                               double *s_dotd, int *s_info) {
-    constexpr int LD = 33;
+/* Makes one wavefront's LDS writes visible to its other lanes: a wavefront-
+   scope release/acquire around the wave barrier, so the compiler neither keeps
+   stale LDS values in registers nor moves accesses across it. */
+__device__ __forceinline__ void bv_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* 64-bit readlane: the value of v in lane `src` (wave-uniform). */
+__device__ __forceinline__ double bv_readlane(double v, int src) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), src);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+/* Y = R J^T for 16x16 tiles in accumulator layout (matrix-core products). */
+__device__ __forceinline__ bv_d4 bv_rjt(const bv_d4 &r, const bv_d4 &jm, bv_d4 acc) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int e = tid + BV_THREADS * t;
-        const int r = e & 31, cc = e >> 5;
-        sR[r * LD + cc] = (r == cc) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    const int km = m - J0; /* local index of the observation row, if in range */
-    for (int k = 0; k < BV_NB; ++k) {
-        const int qk = J0 + k;
-        const double akk = sA[k * LD + k];
-        const double piv = (qk >= m) ? 1.0 : sqrt(akk);
+    for (int g = 0; g < 4; ++g) acc = bv_mfma(jm[g], r[g], acc);
+    return acc;
+}
+
+/* Transpose of a 16x16 tile in accumulator layout, through an LDS scratch of
+   16 x 17 doubles (one wavefront). */
+__device__ __forceinline__ bv_d4 bv_transpose16(const bv_d4 &v, int lane, double *sT) {
+    const int lr = lane & 15, lq = lane >> 4;
+    bv_wave_sync(); /* earlier reads of sT are done */
+#pragma unroll
+    for (int g = 0; g < 4; ++g) sT[lr * 17 + 4 * g + lq] = v[g];
+    bv_wave_sync();
+    bv_d4 t;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) t[g] = sT[(4 * g + lq) * 17 + lr];
+    return t;
+}
+
+/* Unblocked Cholesky of a 16x16 tile held in accumulator layout (lane l holds
+   row l%16, columns 4g + l/16), and in the same sweep W = L^-1 by forward
+   elimination of the identity.  Step k:
+     l_kk = sqrt(a_kk)                     (readlane from lane k + 16 (k%4))
+     l_rk = a_rk / l_kk                    (from the lane of row r holding column k)
+     a_rc -= l_rk l_ck                     (column k broadcast through sCol)
+     W_k = R_k / l_kk; R_r -= l_rk W_k     (r > k; row k broadcast through sRow)
+   Entries with c <= k or r <= k are updated too but never read again; W stays
+   lower triangular.  The lanes exchange values through LDS with a wavefront
+   release/acquire (bv_wave_sync) between the writes and the reads.  kbase: global index of local column 0.  Pivots go to
+   sPiv[pivoff + k]; squares of row m's factor entries in columns >= cs0 add to
+   dacc; the first failing column (global, + 1) to info. */
+__device__ __forceinline__ bv_d4 bv_sweep16(bv_d4 a, int lane, int kbase, int m, int cs0,
+                                            double *sCol, double *sRow, double *sPiv,
+                                            int pivoff, double &dacc, int &info) {
+    const int lr = lane & 15, lq = lane >> 4;
+    const int kml = m - kbase;
+    bv_d4 w;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) w[g] = (4 * g + lq == lr) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int kq = k & 3, kg = k >> 2;
+        const int q = kbase + k;
+        const double akk = bv_readlane(a[kg], k + 16 * kq);
+        const double piv = (q >= m) ? 1.0 : sqrt(akk);
         const double pinv = 1.0 / piv;
-        if (tid == 0) {
-            if (qk < m) {
-                if (!(akk > 0.0) && *s_info == 0) *s_info = qk + 1;
-                if (qk >= cs0) *s_logdet += log(piv * piv);
-            }
-            if (km >= 0 && km < BV_NB && k < km && qk >= cs0) {
-                const double v = sA[k * LD + km] * pinv;
-                *s_dotd += v * v;
+        if (q < m && !(akk > 0.0) && info == 0) info = q + 1;
+        const double t = a[kg] * pinv;
+        const double lrk = __shfl(t, lr + 16 * kq, 64);
+        if (lr == kml && k < kml && q >= cs0 && lq == 0) dacc += lrk * lrk;
+        if (lr == k) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                w[g] *= pinv;
+                sRow[4 * g + lq] = w[g];
             }
         }
+        if (lq == kq) sCol[lr] = lrk;
+        if (lane == 0) sPiv[pivoff + k] = piv;
+        bv_wave_sync();
+        const double lre = (lr > k) ? lrk : 0.0;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int e = tid + BV_THREADS * t;
-            const int r = e & 31, cc = e >> 5;
-            if (cc > k) {
-                if (r >= cc) sA[cc * LD + r] -= (sA[k * LD + r] * pinv) * (sA[k * LD + cc] * pinv);
-            } else if (r == k) {
-                const double w = sR[k * LD + cc] * pinv;
-                const int tileid = (k < 16) ? 0 : (cc < 16 ? 1 : 2);
-                const int rl = k & 15, cl = cc & 15;
-                sW[tileid * 256 + 4 * (rl + 16 * (cl & 3)) + (cl >> 2)] = w;
-            } else if (r > k) {
-                sR[r * LD + cc] -= (sA[k * LD + r] * pinv) * (sR[k * LD + cc] * pinv);
-            }
+        for (int g = 0; g < 4; ++g) {
+            a[g] -= lrk * sCol[4 * g + lq];
+            w[g] -= lre * sRow[4 * g + lq];
         }
-        __syncthreads();
+        bv_wave_sync(); /* reads done before the next step's writes */
     }
-    /* upper parts of the W tiles (0,0) and (1,1) are zero */
+    return w;
+}
+
+/* Phase 2, one wavefront: factor the 32x32 diagonal block whose C tiles are
+   (rt0,rt0), (rt0+1,rt0), (rt0+1,rt0+1) of the workspace, as 2x2 blocks of
+   16x16:  L11, W11 = sweep(A11);  L21 = A21 W11^T;  A22' = A22 - L21 L21^T;
+   L22, W22 = sweep(A22');  W21 = -W22 (L21 W11).  W = L_jj^-1 goes to sW as
+   the tiles (0,0), (1,0), (1,1) in accumulator layout; pivots to sPiv. */
+__device__ __forceinline__ void bv_diag_factor_wave(const double *__restrict__ ws, int T, int rt0,
+                                                    int lane, int J0, int m, int cs0,
+                                                    double *sCol, double *sRow, double *sT,
+                                                    double *sW, double *sPiv, double *s_dotd,
+                                                    int *s_info) {
+    const int lr = lane & 15, lq = lane >> 4;
+    const bv_d4 a11 = *(const bv_d4 *)(ws + bv_tile_off(rt0, rt0, T) + 4 * lane);
+    const bv_d4 a21 = *(const bv_d4 *)(ws + bv_tile_off(rt0 + 1, rt0, T) + 4 * lane);
+    const bv_d4 a22 = *(const bv_d4 *)(ws + bv_tile_off(rt0 + 1, rt0 + 1, T) + 4 * lane);
+    double dacc = 0.0;
+    int info = 0;
+    const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
+    const bv_d4 w11 = bv_sweep16(a11, lane, J0, m, cs0, sCol, sRow, sPiv, 0, dacc, info);
+    const bv_d4 l21 = bv_rjt(a21, w11, zero);
+    const bv_d4 a22p = -bv_rjt(l21, l21, -a22);
+    const int km = m - J0;
+    if (km >= 16 && km < 32 && lr == km - 16) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int e = tid + BV_THREADS * t; /* 0..511: two 16x16 tiles */
-        const int tile = e >> 8, rl = (e >> 4) & 15, cl = e & 15;
-        if (cl > rl) sW[(tile ? 2 : 0) * 256 + 4 * (rl + 16 * (cl & 3)) + (cl >> 2)] = 0.0;
+        for (int g = 0; g < 4; ++g)
+            if (J0 + 4 * g + lq >= cs0) dacc += l21[g] * l21[g];
     }
-    __syncthreads();
+    const bv_d4 w22 = bv_sweep16(a22p, lane, J0 + 16, m, cs0, sCol, sRow, sPiv, 16, dacc, info);
+    const bv_d4 w11t = bv_transpose16(w11, lane, sT);
+    const bv_d4 pm = bv_rjt(l21, w11t, zero);          /* L21 W11 */
+    const bv_d4 pmt = bv_transpose16(pm, lane, sT);
+    const bv_d4 w21 = -bv_rjt(w22, pmt, zero);         /* -W22 L21 W11 */
+    *(bv_d4 *)(sW + 0 * 256 + 4 * lane) = w11;
+    *(bv_d4 *)(sW + 1 * 256 + 4 * lane) = w21;
+    *(bv_d4 *)(sW + 2 * 256 + 4 * lane) = w22;
+    /* fixed-order reduction of row m's squares over the wavefront */
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) dacc += __shfl_xor(dacc, off, 64);
+    if (lane == 0) {
+        *s_dotd += dacc;
+        if (info != 0 && *s_info == 0) *s_info = info;
+    }
 }
 
 template <int KIND>
@@ -305,9 +373,9 @@ __global__ void __launch_bounds__(BV_THREADS, 2)
 bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
                 double *ws_base) {
     extern __shared__ double smem[];
-    __shared__ double sA[BV_NB * 33];
-    __shared__ double sR[BV_NB * 33];
     __shared__ double sW[3 * 256];
+    __shared__ double sT[16 * 17];
+    __shared__ double sCol[16], sRow[16], sPiv[BV_NB];
     __shared__ double s_logdet, s_dotd, s_wdot[BV_THREADS / 64];
     __shared__ int s_info;
 
@@ -356,19 +424,21 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
             bv_pass_compute(c, ws, T, j, wave, lane, i0);
         }
         __syncthreads();
-        /* phase 2: the diagonal block's C tiles -> sA, factor and invert */
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            const int e = tid + BV_THREADS * t;       /* 0..767: tiles (0,0),(1,0),(1,1) */
-            const int tl = e >> 8, off = e & 255;
-            const int l = off >> 2, g = off & 3;
-            const int r = (l & 15) + (tl ? 16 : 0);
-            const int cc = 4 * g + (l >> 4) + (tl == 2 ? 16 : 0);
-            const int trt = rt0 + (tl ? 1 : 0), tct = rt0 + (tl == 2 ? 1 : 0);
-            sA[cc * 33 + r] = ws[bv_tile_off(trt, tct, T) + off];
+        /* phase 2 (wavefront 0): factor and invert the diagonal block */
+        if (wave == 0) {
+            bv_diag_factor_wave(ws, T, rt0, lane, J0, m, cs0, sCol, sRow, sT, sW, sPiv, &s_dotd,
+                                &s_info);
+            /* log-determinant over the block rows: terms in column order */
+            if (lane == 0) {
+                double ld = s_logdet;
+                for (int k = 0; k < BV_NB; ++k) {
+                    const int q = J0 + k;
+                    if (q >= cs0 && q < m) ld += log(sPiv[k] * sPiv[k]);
+                }
+                s_logdet = ld;
+            }
         }
         __syncthreads();
-        bv_diag_factor(tid, J0, m, cs0, sA, sR, sW, &s_logdet, &s_dotd, &s_info);
         /* phase 3: rows below the diagonal block */
         {
             const bv_d4 w00 = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
