@@ -6,8 +6,9 @@
  * blocked Cholesky:
  *   for each 32-column panel j:
  *     0: A(rows >= 32j, panel) generated into the panel's workspace tiles
- *        (closed-form Matern kinds; general nu and space-time entries are
- *        written beforehand by bv_gen_kernel)
+ *        (closed-form Matern kinds: for panel j+1 by wavefronts 1-3 while
+ *        wavefront 0 runs phase 2 of panel j; general nu and space-time
+ *        entries are written beforehand by bv_gen_kernel)
  *     1: C = A - L(rows, 0:32j) L(panel, 0:32j)^T (FP64 matrix-core 16x16x4
  *        products, accumulators in registers), stored over A
  *     2: diagonal 32x32 block of C -> LDS; one sweep computes its Cholesky
@@ -137,19 +138,18 @@ __device__ __forceinline__ void bv_panel_gemm(const double *__restrict__ ws, int
     }
 }
 
-/* Phase 0 of a panel: A(rt, panel) for this wave's row tiles rt0 + wave + 4i,
-   generated from the staged locations and written to the tiles (rt, rt0),
-   (rt, rt0+1) in the lane layout phase 1 reads them back in (each lane reads
-   only what it wrote, so no barrier is needed between the two phases). */
+/* A(rt, panel j) for the row tiles rt = rtb, rtb + rstep, ..., generated from
+   the staged locations and written to the tiles (rt, 2j), (rt, 2j+1) in
+   accumulator layout. */
 template <int KIND>
-__device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T, int j, int wave,
-                                                  int lane, int m, const double *sx,
+__device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T, int j, int rtb,
+                                                  int rstep, int lane, int m, const double *sx,
                                                   const double *sy, const double *sz,
                                                   const double *so, const bv_kparams &kp) {
     const int J0 = j * BV_NB;
     const int rt0 = 2 * j;
     const int lr = lane & 15, lq = lane >> 4;
-    for (int rt = rt0 + wave; rt < T; rt += 4) {
+    for (int rt = rtb; rt < T; rt += rstep) {
         const int p = rt * BV_TILE + lr;
         const double xp = sx[p], yp = sy[p], zp = sz[p];
         for (int ct = 0; ct < 2; ++ct) {
@@ -416,15 +416,23 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
         const int cnt = (nrt > wave) ? (nrt - wave + 3) / 4 : 0;
         const int ngrp = (cnt + BV_NRT - 1) / BV_NRT;
         const int gsz = ngrp ? (cnt + ngrp - 1) / ngrp : 0;
-        if (KIND != BV_KIND_LOADA)
-            bv_panel_generate<KIND>(ws, T, j, wave, lane, m, sx, sy, sz, so, kp);
+        /* phase 0 for the first panel (later panels are generated during the
+           previous panel's phase 2) */
+        if (KIND != BV_KIND_LOADA && j == 0) {
+            bv_panel_generate<KIND>(ws, T, 0, wave, 4, lane, m, sx, sy, sz, so, kp);
+            __syncthreads();
+        }
         for (int gi = 0; gi < ngrp; ++gi) {
             const int i0 = gi * gsz;
             const int c = (cnt - i0) < gsz ? (cnt - i0) : gsz;
             bv_pass_compute(c, ws, T, j, wave, lane, i0);
         }
         __syncthreads();
-        /* phase 2 (wavefront 0): factor and invert the diagonal block */
+        /* phase 2 (wavefront 0): factor and invert the diagonal block, while
+           wavefronts 1-3 generate the next panel's covariance tiles */
+        if (KIND != BV_KIND_LOADA && wave > 0 && j + 1 < P)
+            bv_panel_generate<KIND>(ws, T, j + 1, rt0 + 2 + wave - 1, 3, lane, m, sx, sy, sz, so,
+                                    kp);
         if (wave == 0) {
             bv_diag_factor_wave(ws, T, rt0, lane, J0, m, cs0, sCol, sRow, sT, sW, sPiv, &s_dotd,
                                 &s_info);
