@@ -154,6 +154,7 @@ static int bv_make_kparams(const bv_ctx *c, const double *theta, bv_kparams *kp,
     con = 1.0 / con;
     kp->con = theta[0] * con;
     kp->dist_scale = c->dist_scale;
+    kp->inv_scale = 1.0 / (theta[1] * c->dist_scale);
     kp->metric = c->metric;
     kp->use_z = (c->time_flag && c->d_z) ? 1 : 0;
     /* the reference's box bounds are < 100 (include/vecchia_helper_data.h:684-700);

@@ -24,7 +24,12 @@
 #define BV_NB 32          /* panel width (columns factored per left-looking step) */
 #define BV_TILE 16        /* matrix-core tile edge                               */
 #define BV_THREADS 256    /* 4 wavefronts of 64 lanes per block                  */
+#ifndef BV_NRT
 #define BV_NRT 4          /* row tiles held in accumulators per wave per pass    */
+#endif
+#ifndef BV_WAVES_PER_SIMD
+#define BV_WAVES_PER_SIMD 2 /* occupancy target of the factorization kernel       */
+#endif
 #define BV_MAX_MP 1024    /* largest padded augmented size handled (LDS staging)  */
 
 /* One block of one evaluation, in launch order. */

@@ -149,18 +149,33 @@ __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T
     const int J0 = j * BV_NB;
     const int rt0 = 2 * j;
     const int lr = lane & 15, lq = lane >> 4;
+    /* planar Euclidean closed forms: tiles away from the observation row and
+       the padding take a short path (wave-uniform test) */
+    const bool planar = (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 || KIND == BV_KIND_NU25) &&
+                        kp.metric == 0 && !kp.use_z;
     for (int rt = rtb; rt < T; rt += rstep) {
         const int p = rt * BV_TILE + lr;
         const double xp = sx[p], yp = sy[p], zp = sz[p];
         for (int ct = 0; ct < 2; ++ct) {
             if (rt == rt0 && ct == 1) continue; /* upper tile of the diagonal block */
             bv_d4 v;
+            const int q0 = J0 + ct * BV_TILE;
+            if (planar && rt * BV_TILE + 15 < m && q0 + 15 < m) {
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int q = J0 + ct * BV_TILE + 4 * g + lq;
-                v[g] = bv_aug_entry<KIND>(p, q, m, xp, yp, zp, sx[q], sy[q], sz[q], so[q], kp);
+                for (int g = 0; g < 4; ++g) {
+                    const int q = q0 + 4 * g + lq;
+                    v[g] = bv_matern_closed<KIND>(bv_dist_euclid(xp, yp, 0.0, sx[q], sy[q], 0.0, 0), kp);
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int q = q0 + 4 * g + lq;
+                    v[g] = bv_aug_entry<KIND>(p, q, m, xp, yp, zp, sx[q], sy[q], sz[q], so[q], kp);
+                }
             }
             *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + ct, T) + 4 * lane) = v;
+            /* one tile row at a time: bounds the generation's live registers */
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
@@ -369,7 +384,7 @@ __device__ __forceinline__ void bv_diag_factor_wave(const double *__restrict__ w
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(BV_THREADS, 2)
+__global__ void __launch_bounds__(BV_THREADS, BV_WAVES_PER_SIMD)
 bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
                 double *ws_base) {
     extern __shared__ double smem[];
@@ -378,6 +393,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
     __shared__ double sCol[16], sRow[16], sPiv[BV_NB];
     __shared__ double s_logdet, s_dotd, s_wdot[BV_THREADS / 64];
     __shared__ int s_info;
+    __shared__ bv_kparams s_kp; /* read where used: not held in registers */
 
     const bv_block_desc d = desc[blockIdx.x];
     const bv_eval_slot &E = slots[d.eval];
@@ -386,7 +402,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
     const int m = d.m, cs0 = d.cs0, Mp = d.mp;
     const int T = Mp / BV_TILE, P = Mp / BV_NB;
     double *sx = smem, *sy = sx + Mp, *sz = sy + Mp, *so = sz + Mp;
-    const bv_kparams kp = E.kp;
+    const bv_kparams &kp = s_kp;
 
     if (KIND != BV_KIND_LOADA) {
         /* stage the block's points: conditioning set first, then the block */
@@ -396,7 +412,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
             sx[p] = xv; sy[p] = yv; sz[p] = zv; so[p] = ov;
         }
     }
-    if (tid0 == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; }
+    if (tid0 == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; s_kp = E.kp; }
     __syncthreads();
 
     double *ws = ws_base + d.ws_off;

@@ -54,6 +54,7 @@ struct bv_kparams {
     double nu;          /* theta[2] */
     double con;         /* sigma2 / (2^(nu-1) Gamma(nu)), as src/ckernel.cpp:1072-1074 */
     double dist_scale;
+    double inv_scale;   /* 1 / (beta * dist_scale) */
     double beta_t, nu_t, sep, aux;   /* space-time theta[3..6] */
     int32_t metric;     /* 0 Euclidean, 1 great circle */
     int32_t use_z;      /* 3-D distance */
@@ -168,6 +169,19 @@ BV_HD double bv_dist_euclid(double x1, double y1, double z1, double x2, double y
     return sqrt(s);
 }
 
+/* Closed-form Matern of a distance d for nu = 1/2, 3/2, 5/2 (KIND). */
+template <int KIND>
+BV_HD double bv_matern_closed(double d, const bv_kparams &kp) {
+    if (d == 0.0) return kp.sigma2;
+    /* the reference divides by beta, then by dist_scale; the closed forms
+       multiply by the precomputed reciprocal of their product (within 2 ulp) */
+    const double e = d * kp.inv_scale;
+    const double x = exp(-e);
+    if (KIND == BV_KIND_NU05) return kp.sigma2 * x;
+    if (KIND == BV_KIND_NU15) return kp.sigma2 * (1.0 + e) * x;
+    return kp.sigma2 * (1.0 + e + e * e * (1.0 / 3.0)) * x;
+}
+
 /* One covariance entry between locations (x1,y1,z1) and (x2,y2,z2). */
 template <int KIND>
 BV_HD double bv_cov(double x1, double y1, double z1, double x2, double y2, double z2,
@@ -188,12 +202,11 @@ BV_HD double bv_cov(double x1, double y1, double z1, double x2, double y2, doubl
     double d;
     if (kp.metric == 1) d = bv_dist_earth(x1, y1, x2, y2);
     else d = bv_dist_euclid(x1, y1, z1, x2, y2, z2, kp.use_z);
+    if (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 || KIND == BV_KIND_NU25)
+        return bv_matern_closed<KIND>(d, kp);
     double e = d / kp.beta;
     e = e / kp.dist_scale;
     if (e == 0.0) return kp.sigma2;
-    if (KIND == BV_KIND_NU05) return kp.sigma2 * exp(-e);
-    if (KIND == BV_KIND_NU15) return kp.sigma2 * (1.0 + e) * exp(-e);
-    if (KIND == BV_KIND_NU25) return kp.sigma2 * (1.0 + e + e * e / 3.0) * exp(-e);
     return kp.con * pow(e, kp.nu) * bv_besselk(kp.nu, e);
 }
 

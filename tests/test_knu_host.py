@@ -21,6 +21,7 @@ extern "C" double shim_cov(int kind, double x1, double y1, double x2, double y2,
                            double beta, double nu, double con) {
     bv_kparams kp = {};
     kp.sigma2 = s2; kp.beta = beta; kp.nu = nu; kp.con = con; kp.dist_scale = 1.0;
+    kp.inv_scale = 1.0 / beta;
     switch (kind) {
     case 0: return bv_cov<BV_KIND_NU05>(x1, y1, 0, x2, y2, 0, kp);
     case 1: return bv_cov<BV_KIND_NU15>(x1, y1, 0, x2, y2, 0, kp);
