@@ -22,6 +22,8 @@
 
 hipError_t bv_launch_block(int kind, unsigned nblk, size_t lds, hipStream_t st,
                            const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
+hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t st,
+                              const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
 hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,
                          const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
 hipError_t bv_launch_reduce(unsigned nevals, hipStream_t st, const bv_eval_slot *slots);
@@ -72,6 +74,7 @@ struct bv_ctx {
     int last_k = 0;
     double last_ms = 0.0;
     int iterations = 0;
+    int variant = 0; /* factorization kernel: 0 = four wavefronts per block, 1 = one */
 };
 
 static int bv_nparams(int kernel) {
@@ -200,6 +203,8 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     c->time_flag = p->time_flag || p->kernel == BV_KERNEL_SPACETIME;
     c->dist_scale = p->dist_scale > 0 ? p->dist_scale : 1.0;
     c->verbose = p->verbose;
+    /* factorization kernel: BV_FACTOR_KERNEL=1w selects the one-wavefront one */
+    if (const char *v = getenv("BV_FACTOR_KERNEL")) c->variant = (strcmp(v, "1w") == 0) ? 1 : 0;
     c->h_batch_num.assign(p->batch_num, p->batch_num + p->batch_count);
     c->h_accum.assign((size_t)c->B + 1, 0);
     for (int i = 0; i < c->B; ++i) {
@@ -316,7 +321,10 @@ static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *ou
             BV_HIP(bv_launch_gen(kind, T * (T + 1) / 2, nblk, c->stream, c->d_desc, c->d_slots,
                                  c->d_ws));
         }
-        BV_HIP(bv_launch_block(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
+        if (c->variant == 1)
+            BV_HIP(bv_launch_block_1w(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
+        else
+            BV_HIP(bv_launch_block(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
     }
     BV_HIP(hipEventRecord(c->ev1, c->stream));
     BV_HIP(bv_launch_reduce((unsigned)k, c->stream, c->d_slots));

@@ -99,17 +99,17 @@ __device__ __forceinline__ void bv_gemm_step(const bv_d4 &j0, const bv_d4 &j1, c
 }
 
 template <int C>
-__device__ __forceinline__ void bv_gemm_load(const double *colp, int rt0, int wave, int i0, int lo,
-                                             bv_d4 &j0, bv_d4 &j1, bv_d4 (&r)[C]) {
+__device__ __forceinline__ void bv_gemm_load(const double *colp, int rt0, int rbase, int rstride,
+                                             int lo, bv_d4 &j0, bv_d4 &j1, bv_d4 (&r)[C]) {
     j0 = *(const bv_d4 *)(colp + ((int64_t)rt0 << 8) + lo);
     j1 = *(const bv_d4 *)(colp + ((int64_t)(rt0 + 1) << 8) + lo);
 #pragma unroll
     for (int s = 0; s < C; ++s)
-        r[s] = *(const bv_d4 *)(colp + ((int64_t)(rt0 + wave + 4 * (i0 + s)) << 8) + lo);
+        r[s] = *(const bv_d4 *)(colp + ((int64_t)(rbase + rstride * s) << 8) + lo);
 }
 
 /* acc += L(panel rows ct) L(row tile s)^T over the first 2j tile columns, for
-   the C row tiles rt0 + wave + 4 (i0 + s).  C is a compile-time count so the
+   the C row tiles rbase + rstride s.  C is a compile-time count so the
    loop is straight-line matrix-core code.  Two operand buffers alternate: the
    loads of tile column kt+1 are in flight while the products of column kt
    issue, with no register copies between iterations.  Tile (rt, kt) sits at
@@ -117,22 +117,22 @@ __device__ __forceinline__ void bv_gemm_load(const double *colp, int rt0, int wa
    advances by T - kt - 1 tiles per column, the lane part is 4 * lane. */
 template <int C>
 __device__ __forceinline__ void bv_panel_gemm(const double *__restrict__ ws, int T, int j,
-                                              int wave, int lane, int i0,
+                                              int rbase, int rstride, int lane,
                                               bv_d4 (&acc)[BV_NRT][2]) {
     const int rt0 = 2 * j;
     if (rt0 == 0) return;
     const int lo = 4 * lane;
     const double *colp = ws;
     bv_d4 ja0, ja1, ra[C], jb0, jb1, rb[C];
-    bv_gemm_load<C>(colp, rt0, wave, i0, lo, ja0, ja1, ra);
+    bv_gemm_load<C>(colp, rt0, rbase, rstride, lo, ja0, ja1, ra);
     for (int kt = 0; kt < rt0; kt += 2) {
         /* column kt+1 (rt0 is even, so it always exists here) */
         const double *colb = colp + ((int64_t)(T - kt - 1) << 8);
-        bv_gemm_load<C>(colb, rt0, wave, i0, lo, jb0, jb1, rb);
+        bv_gemm_load<C>(colb, rt0, rbase, rstride, lo, jb0, jb1, rb);
         bv_gemm_step<C>(ja0, ja1, ra, acc);
         /* column kt+2, or column kt+1 again after the last pair */
         const double *cola = (kt + 2 < rt0) ? colb + ((int64_t)(T - kt - 2) << 8) : colb;
-        bv_gemm_load<C>(cola, rt0, wave, i0, lo, ja0, ja1, ra);
+        bv_gemm_load<C>(cola, rt0, rbase, rstride, lo, ja0, ja1, ra);
         bv_gemm_step<C>(jb0, jb1, rb, acc);
         colp = cola;
     }
@@ -183,34 +183,40 @@ __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T
 /* Phase 1 of a panel for C row tiles: acc = A - products, stored as C into
    the tiles (rt, rt0) and (rt, rt0+1) (the slots L will take). */
 template <int C>
-__device__ __forceinline__ void bv_pass_compute_c(double *__restrict__ ws, int T, int j, int wave,
-                                                  int lane, int i0) {
+__device__ __forceinline__ void bv_pass_compute_c(double *__restrict__ ws, int T, int j, int rbase,
+                                                  int rstride, int lane) {
     bv_d4 acc[BV_NRT][2];
     const int rt0 = 2 * j;
     /* acc = -A; the products are added; C = -acc (negations are exact) */
 #pragma unroll
     for (int s = 0; s < C; ++s) {
-        const int rt = rt0 + wave + 4 * (i0 + s);
+        const int rt = rbase + rstride * s;
         acc[s][0] = -*(const bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane);
         acc[s][1] = (rt > rt0) ? -*(const bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane)
                                : (bv_d4){0.0, 0.0, 0.0, 0.0};
     }
-    bv_panel_gemm<C>(ws, T, j, wave, lane, i0, acc);
+    bv_panel_gemm<C>(ws, T, j, rbase, rstride, lane, acc);
 #pragma unroll
     for (int s = 0; s < C; ++s) {
-        const int rt = rt0 + wave + 4 * (i0 + s);
+        const int rt = rbase + rstride * s;
         *(bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane) = -acc[s][0];
         if (rt > rt0) *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane) = -acc[s][1];
     }
 }
 
 __device__ __forceinline__ void bv_pass_compute(int c, double *__restrict__ ws, int T, int j,
-                                                int wave, int lane, int i0) {
+                                                int rbase, int rstride, int lane) {
     switch (c) {
-    case 1: bv_pass_compute_c<1>(ws, T, j, wave, lane, i0); break;
-    case 2: bv_pass_compute_c<2>(ws, T, j, wave, lane, i0); break;
-    case 3: bv_pass_compute_c<3>(ws, T, j, wave, lane, i0); break;
-    case 4: bv_pass_compute_c<4>(ws, T, j, wave, lane, i0); break;
+    case 1: bv_pass_compute_c<1>(ws, T, j, rbase, rstride, lane); break;
+#if BV_NRT >= 2
+    case 2: bv_pass_compute_c<2>(ws, T, j, rbase, rstride, lane); break;
+#endif
+#if BV_NRT >= 3
+    case 3: bv_pass_compute_c<3>(ws, T, j, rbase, rstride, lane); break;
+#endif
+#if BV_NRT >= 4
+    case 4: bv_pass_compute_c<4>(ws, T, j, rbase, rstride, lane); break;
+#endif
     default: break;
     }
 }
@@ -297,7 +303,8 @@ __device__ __forceinline__ bv_d4 bv_transpose16(const bv_d4 &v, int lane, double
      W_k = R_k / l_kk; R_r -= l_rk W_k     (r > k; row k broadcast through sRow)
    Entries with c <= k or r <= k are updated too but never read again; W stays
    lower triangular.  The lanes exchange values through LDS with a wavefront
-   release/acquire (bv_wave_sync) between the writes and the reads.  kbase: global index of local column 0.  Pivots go to
+   release/acquire (bv_wave_sync) between the writes and the reads.  kbase:
+   global index of local column 0.  The squared pivots a_kk = l_kk^2 go to
    sPiv[pivoff + k]; squares of row m's factor entries in columns >= cs0 add to
    dacc; the first failing column (global, + 1) to info. */
 __device__ __forceinline__ bv_d4 bv_sweep16(bv_d4 a, int lane, int kbase, int m, int cs0,
@@ -313,8 +320,17 @@ __device__ __forceinline__ bv_d4 bv_sweep16(bv_d4 a, int lane, int kbase, int m,
         const int kq = k & 3, kg = k >> 2;
         const int q = kbase + k;
         const double akk = bv_readlane(a[kg], k + 16 * kq);
-        const double piv = (q >= m) ? 1.0 : sqrt(akk);
-        const double pinv = 1.0 / piv;
+        /* 1/l_kk = 1/sqrt(a_kk): hardware reciprocal square root refined by two
+           Newton steps y <- y + y (1/2 - (a/2) y^2) (each doubles the correct
+           bits; the hardware estimate carries more than 2^-27) */
+        double pinv = 1.0;
+        if (q < m) {
+            const double ha = 0.5 * akk;
+            double y = __builtin_amdgcn_rsq(akk);
+            y = fma(y, fma(-ha * y, y, 0.5), y);
+            y = fma(y, fma(-ha * y, y, 0.5), y);
+            pinv = y;
+        }
         if (q < m && !(akk > 0.0) && info == 0) info = q + 1;
         const double t = a[kg] * pinv;
         const double lrk = __shfl(t, lr + 16 * kq, 64);
@@ -327,7 +343,7 @@ __device__ __forceinline__ bv_d4 bv_sweep16(bv_d4 a, int lane, int kbase, int m,
             }
         }
         if (lq == kq) sCol[lr] = lrk;
-        if (lane == 0) sPiv[pivoff + k] = piv;
+        if (lane == 0) sPiv[pivoff + k] = akk; /* l_kk^2: log-determinant term */
         bv_wave_sync();
         const double lre = (lr > k) ? lrk : 0.0;
 #pragma unroll
@@ -338,6 +354,22 @@ __device__ __forceinline__ bv_d4 bv_sweep16(bv_d4 a, int lane, int kbase, int m,
         bv_wave_sync(); /* reads done before the next step's writes */
     }
     return w;
+}
+
+/* Log-determinant terms of one panel: log(l_kk^2) for the block rows
+   (cs0 <= J0 + k < m), one per lane 0..31, summed in a fixed tree order and
+   added to *s_logdet by lane 0.  One wavefront. */
+__device__ __forceinline__ void bv_panel_logdet(int lane, int J0, int m, int cs0,
+                                                const double *sPiv, double *s_logdet) {
+    bv_wave_sync();
+    double t = 0.0;
+    if (lane < BV_NB) {
+        const int q = J0 + lane;
+        if (q >= cs0 && q < m) t = log(sPiv[lane]);
+    }
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0) *s_logdet += t;
 }
 
 /* Phase 2, one wavefront: factor the 32x32 diagonal block whose C tiles are
@@ -441,7 +473,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
         for (int gi = 0; gi < ngrp; ++gi) {
             const int i0 = gi * gsz;
             const int c = (cnt - i0) < gsz ? (cnt - i0) : gsz;
-            bv_pass_compute(c, ws, T, j, wave, lane, i0);
+            bv_pass_compute(c, ws, T, j, rt0 + wave + 4 * i0, 4, lane);
         }
         __syncthreads();
         /* phase 2 (wavefront 0): factor and invert the diagonal block, while
@@ -452,15 +484,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
         if (wave == 0) {
             bv_diag_factor_wave(ws, T, rt0, lane, J0, m, cs0, sCol, sRow, sT, sW, sPiv, &s_dotd,
                                 &s_info);
-            /* log-determinant over the block rows: terms in column order */
-            if (lane == 0) {
-                double ld = s_logdet;
-                for (int k = 0; k < BV_NB; ++k) {
-                    const int q = J0 + k;
-                    if (q >= cs0 && q < m) ld += log(sPiv[k] * sPiv[k]);
-                }
-                s_logdet = ld;
-            }
+            bv_panel_logdet(lane, J0, m, cs0, sPiv, &s_logdet);
         }
         __syncthreads();
         /* phase 3: rows below the diagonal block */
@@ -484,6 +508,83 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
         for (int w = 0; w < BV_THREADS / 64; ++w) dsum += s_wdot[w];
         E.logdet[d.block] = s_logdet;
         E.dot[d.block] = dsum;
+        E.info[d.block] = s_info;
+    }
+}
+
+/* bv_block_kernel_1w: one wavefront per block.  The same four phases as
+   bv_block_kernel, all run by the one wavefront: in the tile layout every lane
+   reads back only the workspace values it wrote itself (the cross-lane mixing
+   happens inside the matrix-core products), so no barrier is needed; the
+   diagonal factorization exchanges values through LDS with wavefront syncs.
+   Many blocks per SIMD run concurrently, so one block's serial diagonal
+   factorization overlaps other blocks' matrix-core work. */
+#ifndef BV_1W_WAVES_PER_SIMD
+#define BV_1W_WAVES_PER_SIMD 2
+#endif
+template <int KIND>
+__global__ void __launch_bounds__(64, BV_1W_WAVES_PER_SIMD)
+bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
+                   double *ws_base) {
+    extern __shared__ double smem[];
+    __shared__ double sW[3 * 256];
+    __shared__ double sT[16 * 17];
+    __shared__ double sCol[16], sRow[16], sPiv[BV_NB];
+    __shared__ double s_logdet, s_dotd;
+    __shared__ int s_info;
+    __shared__ bv_kparams s_kp;
+
+    const bv_block_desc d = desc[blockIdx.x];
+    const bv_eval_slot &E = slots[d.eval];
+    const int lane0 = threadIdx.x;
+    const int m = d.m, cs0 = d.cs0, Mp = d.mp;
+    const int T = Mp / BV_TILE, P = Mp / BV_NB;
+    double *sx = smem, *sy = sx + Mp, *sz = sy + Mp, *so = sz + Mp;
+    const bv_kparams &kp = s_kp;
+
+    if (KIND != BV_KIND_LOADA) {
+        for (int p = lane0; p < Mp; p += 64) {
+            double xv, yv, zv, ov;
+            bv_point(d, E, p, xv, yv, zv, ov);
+            sx[p] = xv; sy[p] = yv; sz[p] = zv; so[p] = ov;
+        }
+    }
+    if (lane0 == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; s_kp = E.kp; }
+    bv_wave_sync();
+
+    double *ws = ws_base + d.ws_off;
+    double dot_lane = 0.0;
+    for (int j = 0; j < P; ++j) {
+        int lane = lane0;
+        asm volatile("" : "+v"(lane)); /* recompute lane-derived values per panel */
+        const int J0 = j * BV_NB;
+        const int rt0 = 2 * j;
+        const int nrt = T - rt0;
+        if (KIND != BV_KIND_LOADA)
+            bv_panel_generate<KIND>(ws, T, j, rt0, 1, lane, m, sx, sy, sz, so, kp);
+        const int ngrp = (nrt + BV_NRT - 1) / BV_NRT;
+        const int gsz = (nrt + ngrp - 1) / ngrp;
+        for (int gi = 0; gi < ngrp; ++gi) {
+            const int i0 = gi * gsz;
+            const int c = (nrt - i0) < gsz ? (nrt - i0) : gsz;
+            bv_pass_compute(c, ws, T, j, rt0 + i0, 1, lane);
+        }
+        bv_diag_factor_wave(ws, T, rt0, lane, J0, m, cs0, sCol, sRow, sT, sW, sPiv, &s_dotd,
+                            &s_info);
+        bv_panel_logdet(lane, J0, m, cs0, sPiv, &s_logdet);
+        const bv_d4 w00 = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
+        const bv_d4 w10 = *(const bv_d4 *)(sW + 1 * 256 + 4 * lane);
+        const bv_d4 w11 = *(const bv_d4 *)(sW + 2 * 256 + 4 * lane);
+        for (int rt = rt0 + 2; rt < T; ++rt)
+            bv_tile_solve(ws, T, j, rt, lane, m, cs0, w00, w10, w11, dot_lane);
+        bv_wave_sync();
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) dot_lane += __shfl_xor(dot_lane, off, 64);
+    bv_wave_sync();
+    if (lane0 == 0) {
+        E.logdet[d.block] = s_logdet;
+        E.dot[d.block] = s_dotd + dot_lane;
         E.info[d.block] = s_info;
     }
 }
@@ -579,6 +680,29 @@ bv_selftest_kernel(const double *a, const double *b, double *d) {
 }
 
 /* ---- host-side launch wrappers (called from bv_api.hip) ---- */
+hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t st,
+                              const bv_block_desc *desc, const bv_eval_slot *slots, double *ws) {
+    switch (kind) {
+    case BV_KIND_NU05:
+        hipLaunchKernelGGL(bv_block_kernel_1w<BV_KIND_NU05>, dim3(nblk), dim3(64), lds, st, desc,
+                           slots, ws);
+        break;
+    case BV_KIND_NU15:
+        hipLaunchKernelGGL(bv_block_kernel_1w<BV_KIND_NU15>, dim3(nblk), dim3(64), lds, st, desc,
+                           slots, ws);
+        break;
+    case BV_KIND_NU25:
+        hipLaunchKernelGGL(bv_block_kernel_1w<BV_KIND_NU25>, dim3(nblk), dim3(64), lds, st, desc,
+                           slots, ws);
+        break;
+    default:
+        hipLaunchKernelGGL(bv_block_kernel_1w<BV_KIND_LOADA>, dim3(nblk), dim3(64), 0, st, desc,
+                           slots, ws);
+        break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t bv_launch_block(int kind, unsigned nblk, size_t lds, hipStream_t st,
                            const bv_block_desc *desc, const bv_eval_slot *slots, double *ws) {
     switch (kind) {
