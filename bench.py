@@ -14,6 +14,18 @@ kernel's FP64 rate, HIP-event timed) and `cpu_baseline` (the CPU port of the
 same chain, timed on this host's cores).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
+
+HBM traffic of the factorization kernel (the `traffic` field) comes from two
+rocprofv3 counter passes over this script, each with a plain device copy of
+known size appended as the calibration (--calib-copy-mib), summarised by
+--traffic-from into profiles/bv_block_kernel_n20000_cs300_traffic.json:
+
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- \
+        python bench.py --steps 5 --warmup 1 --no-cpu-baseline --calib-copy-mib 512
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- \
+        python bench.py --steps 5 --warmup 1 --no-cpu-baseline --calib-copy-mib 512
+    python bench.py --traffic-from OUT/fetch/run_counter_collection.csv \
+        OUT/write/run_counter_collection.csv --calib-copy-mib 512
 """
 import argparse
 import json
@@ -65,6 +77,74 @@ def traffic_from_profiles():
     return d.get("hbm_bytes_per_launch")
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bv_block_kernel_n20000_cs300_traffic.json")
+
+
+def calibration_copy(mib, repeats=3):
+    """A plain streaming copy of `mib` MiB on the device (an elementwise
+    kernel, so the counters see it; a same-dtype copy_ may go to a DMA
+    engine): the known-size reference for FETCH_SIZE / WRITE_SIZE.  These are
+    the last three elementwise dispatches of the run."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(repeats):
+        torch.mul(src, 1.0, out=dst)  # a streaming kernel: reads n doubles, writes n
+    torch.cuda.synchronize()
+
+
+def summarize_traffic(fetch_csv, write_csv, mib):
+    """Per-dispatch FETCH_SIZE and WRITE_SIZE (KiB) of bv_block_kernel and of
+    the calibration copy, from rocprofv3 counter CSVs; the kernel's HBM bytes
+    are the counters scaled by copy bytes / copy counter."""
+    import csv
+
+    def per_dispatch(path, counter):
+        vals = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"]
+            if name.startswith("void bv_block_kernel<"):
+                key = "kernel"
+            elif "copy" in name.lower() or "elementwise" in name.lower():
+                key = "copy"
+            else:
+                continue
+            did = int(r["Dispatch_Id"])
+            vals.setdefault(key, {}).setdefault(did, 0.0)
+            vals[key][did] += float(r["Counter_Value"])
+        out = {}
+        for key, d in vals.items():
+            ids = sorted(d)
+            if key == "copy":
+                ids = ids[-3:]  # the calibration copies run last
+            out[key] = sum(d[i] for i in ids) / len(ids)
+        return out
+
+    f = per_dispatch(fetch_csv, "FETCH_SIZE")
+    w = per_dispatch(write_csv, "WRITE_SIZE")
+    copy_bytes = mib * (1 << 20)
+    f_scale = copy_bytes / (f["copy"] * 1024.0)
+    w_scale = copy_bytes / (w["copy"] * 1024.0)
+    fetch = f["kernel"] * 1024.0 * f_scale
+    write = w["kernel"] * 1024.0 * w_scale
+    d = {
+        "kernel": "bv_block_kernel<0> (nu=1/2), n=20000 bc=1500 cs=300, 1478 blocks",
+        "fetch_kib_per_dispatch": f["kernel"], "write_kib_per_dispatch": w["kernel"],
+        "copy_bytes": copy_bytes,
+        "copy_fetch_kib": f["copy"], "copy_write_kib": w["copy"],
+        "fetch_scale": f_scale, "write_scale": w_scale,
+        "hbm_fetch_bytes_per_launch": fetch, "hbm_write_bytes_per_launch": write,
+        "hbm_bytes_per_launch": fetch + write,
+    }
+    with open(TRAFFIC_JSON, "w") as fh:
+        json.dump(d, fh, indent=1)
+    print(json.dumps(d, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,7 +152,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=5)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="append a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise two rocprofv3 counter CSVs into the traffic JSON and exit")
     args = ap.parse_args()
+    if args.traffic_from:
+        summarize_traffic(args.traffic_from[0], args.traffic_from[1], args.calib_copy_mib or 512)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the likelihood shards by blocks "
               "across GPUs with a single 8-byte sum; see DESIGN.md); --gpus %d is not run"
@@ -110,6 +197,8 @@ def main():
     if torch is not None and torch.cuda.is_available():
         torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    if args.calib_copy_mib:
+        calibration_copy(args.calib_copy_mib)
     evals_per_s = args.steps / elapsed
     lat_ms = np.array(lat) * 1e3
     kern_ms = float(np.mean(kms))
