@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=5)
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving scenarios (many sessions on resident datasets)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="append a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -226,6 +228,17 @@ def main():
                          "OpenMP over blocks)" % (blocks.batch_count, args.cpu_repeats)}
         log("cpu port %.3f s/eval on %d threads, gpu vs cpu rel err %.3e" % (med, threads, rel_err))
 
+    serving = None
+    if not args.no_serving:
+        from blockvecchia_amd import serving as srv
+        t0 = time.perf_counter()
+        serving = srv.scenarios()
+        for name, r in serving.items():
+            log("serving %s: %.0f evals/s vs plain %.0f (x%.2f), p50 %.2f ms p99 %.2f ms" %
+                (name, r["evals_per_s"], r["plain_evals_per_s"], r["speedup_vs_plain"],
+                 r["latency_ms"]["p50"], r["latency_ms"]["p99"]))
+        log("serving scenarios: %.1f s" % (time.perf_counter() - t0))
+
     traffic = traffic_from_profiles()
     out = {
         "metric": METRIC,
@@ -262,6 +275,7 @@ def main():
             "kernel_ms": kern_ms,
         },
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     eng.close()
     print(json.dumps(out), flush=True)

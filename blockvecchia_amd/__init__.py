@@ -80,6 +80,8 @@ EXPORTED = [
     "bv_create", "bv_llh", "llh_Xvecchia_batch", "bv_llh_many", "bv_block_terms",
     "bv_last_status", "bv_last_kernel_ms", "bv_iterations", "bv_destroy", "bv_matern",
     "bv_partition", "bv_blocks_free", "bv_device_selftest",
+    "bv_server_create", "bv_server_attach", "bv_server_submit", "bv_server_poll",
+    "bv_server_pending", "bv_server_destroy",
 ]
 
 _lib = None
@@ -124,6 +126,21 @@ def load_library(path=None):
     lib.bv_blocks_free.restype = None
     lib.bv_device_selftest.argtypes = [ctypes.c_int]
     lib.bv_device_selftest.restype = ctypes.c_int
+    lib.bv_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_void_p)]
+    lib.bv_server_create.restype = ctypes.c_int
+    lib.bv_server_attach.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    lib.bv_server_attach.restype = ctypes.c_int
+    lib.bv_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp,
+                                     ctypes.c_int64]
+    lib.bv_server_submit.restype = ctypes.c_int
+    lib.bv_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int64), _dp, ctypes.POINTER(ctypes.c_int)]
+    lib.bv_server_poll.restype = ctypes.c_int
+    lib.bv_server_pending.argtypes = [ctypes.c_void_p]
+    lib.bv_server_pending.restype = ctypes.c_int
+    lib.bv_server_destroy.argtypes = [ctypes.c_void_p]
+    lib.bv_server_destroy.restype = None
     if path is None:
         _lib = lib
     return lib
@@ -267,6 +284,61 @@ class Engine:
         if getattr(self, "ctx", None):
             self._lib.bv_destroy(self.ctx)
             self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server over several resident datasets (Engine objects):
+    submit(dataset_id, theta, tag) queues an evaluation; poll() starts queued
+    work on idle lanes and returns finished (tag, llh, status) triples."""
+
+    def __init__(self, lanes=2, ws_bytes_per_lane=2 << 30, max_evals_per_batch=64, device=0):
+        self._lib = load_library()
+        srv = ctypes.c_void_p()
+        rc = self._lib.bv_server_create(device, lanes, float(ws_bytes_per_lane),
+                                        max_evals_per_batch, ctypes.byref(srv))
+        if rc != BV_OK:
+            raise RuntimeError("bv_server_create failed: %d" % rc)
+        self.srv = srv
+        self.engines = []
+        self._cap = 4096
+        self._tags = (ctypes.c_int64 * self._cap)()
+        self._llh = np.empty(self._cap, np.float64)
+        self._st = (ctypes.c_int * self._cap)()
+
+    def attach(self, engine):
+        did = ctypes.c_int(-1)
+        rc = self._lib.bv_server_attach(self.srv, engine.ctx, ctypes.byref(did))
+        if rc != BV_OK:
+            raise RuntimeError("bv_server_attach failed: %d" % rc)
+        self.engines.append(engine)
+        return did.value
+
+    def submit(self, dataset_id, theta, tag):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        rc = self._lib.bv_server_submit(self.srv, dataset_id, len(th), _d(th), int(tag))
+        if rc != BV_OK:
+            raise RuntimeError("bv_server_submit failed: %d" % rc)
+
+    def poll(self, wait=False):
+        n = self._lib.bv_server_poll(self.srv, 1 if wait else 0, self._cap, self._tags,
+                                     _d(self._llh), self._st)
+        if n < 0:
+            raise RuntimeError("bv_server_poll failed: %d" % n)
+        return [(int(self._tags[i]), float(self._llh[i]), int(self._st[i])) for i in range(n)]
+
+    def pending(self):
+        return self._lib.bv_server_pending(self.srv)
+
+    def close(self):
+        if getattr(self, "srv", None):
+            self._lib.bv_server_destroy(self.srv)
+            self.srv = None
 
     def __del__(self):
         try:

@@ -490,3 +490,286 @@ extern "C" int bv_device_selftest(int device) {
         }
     return BV_OK;
 }
+
+/* ======================================================================
+ * Request server: many datasets, many concurrent evaluations.
+ *
+ * Each dataset is a bv_ctx (resident in HBM).  Requests (dataset, theta, tag)
+ * queue in arrival order.  A lane is one stream with its own workspace,
+ * descriptor, slot and result buffers; when a lane is idle, bv_server_poll
+ * admits queued requests into it while their estimated workspace bytes fit
+ * the lane's budget (at least one request per batch), merges the blocks of
+ * all admitted evaluations -- across datasets -- into one launch per
+ * covariance kind (blocks sorted by padded size, largest first), and starts
+ * it without waiting.  Several lanes run at once, so requests that arrive
+ * while a batch is on the GPU are admitted as soon as a lane frees, and their
+ * work overlaps on the device.  Every evaluation uses the same kernels and
+ * fixed-order reductions as bv_llh, so its result does not depend on the
+ * batch it ran in.
+ * ====================================================================== */
+struct bv_request {
+    int dataset;
+    int64_t tag;
+    double theta[8];
+};
+
+struct bv_lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<bv_request> reqs;            /* the batch in flight          */
+    /* capacities (grown while the lane is idle) */
+    int64_t cap_ws = 0, cap_blocks = 0;
+    int cap_evals = 0;
+    double *d_ws = nullptr;
+    bv_block_desc *d_desc = nullptr, *h_desc = nullptr;
+    bv_eval_slot *d_slots = nullptr, *h_slots = nullptr;
+    double *d_logdet = nullptr, *d_dot = nullptr, *d_result = nullptr, *h_result = nullptr;
+    int32_t *d_info = nullptr;
+};
+
+struct bv_server {
+    int device = 0;
+    double ws_budget = 0.0;                  /* bytes of workspace per lane  */
+    int max_evals = 64;                      /* evaluations per batch        */
+    std::vector<bv_ctx *> datasets;
+    std::vector<bv_request> queue;           /* FIFO (front = oldest)        */
+    size_t qhead = 0;
+    std::vector<bv_lane> lanes;
+};
+
+static void bv_lane_free(bv_lane &L) {
+    if (L.d_ws) hipFree(L.d_ws);
+    if (L.d_desc) hipFree(L.d_desc);
+    if (L.h_desc) hipHostFree(L.h_desc);
+    if (L.d_slots) hipFree(L.d_slots);
+    if (L.h_slots) hipHostFree(L.h_slots);
+    if (L.d_logdet) hipFree(L.d_logdet);
+    if (L.d_dot) hipFree(L.d_dot);
+    if (L.d_info) hipFree(L.d_info);
+    if (L.d_result) hipFree(L.d_result);
+    if (L.h_result) hipHostFree(L.h_result);
+    L.d_ws = nullptr; L.d_desc = nullptr; L.h_desc = nullptr; L.d_slots = nullptr;
+    L.h_slots = nullptr; L.d_logdet = nullptr; L.d_dot = nullptr; L.d_info = nullptr;
+    L.d_result = nullptr; L.h_result = nullptr;
+    L.cap_ws = L.cap_blocks = 0;
+    L.cap_evals = 0;
+}
+
+static int bv_lane_reserve(bv_lane &L, int64_t ws, int64_t blocks, int evals) {
+    if (ws <= L.cap_ws && blocks <= L.cap_blocks && evals <= L.cap_evals) return BV_OK;
+    ws = std::max(ws, L.cap_ws);
+    blocks = std::max(blocks, L.cap_blocks);
+    evals = std::max(evals, L.cap_evals);
+    bv_lane_free(L);
+    BV_HIP(hipMalloc((void **)&L.d_ws, sizeof(double) * (size_t)ws));
+    BV_HIP(hipMalloc((void **)&L.d_desc, sizeof(bv_block_desc) * (size_t)blocks));
+    BV_HIP(hipHostMalloc((void **)&L.h_desc, sizeof(bv_block_desc) * (size_t)blocks,
+                         hipHostMallocDefault));
+    BV_HIP(hipMalloc((void **)&L.d_slots, sizeof(bv_eval_slot) * evals));
+    BV_HIP(hipHostMalloc((void **)&L.h_slots, sizeof(bv_eval_slot) * evals, hipHostMallocDefault));
+    BV_HIP(hipMalloc((void **)&L.d_logdet, sizeof(double) * (size_t)blocks));
+    BV_HIP(hipMalloc((void **)&L.d_dot, sizeof(double) * (size_t)blocks));
+    BV_HIP(hipMalloc((void **)&L.d_info, sizeof(int32_t) * (size_t)blocks));
+    BV_HIP(hipMalloc((void **)&L.d_result, sizeof(double) * 4 * evals));
+    BV_HIP(hipHostMalloc((void **)&L.h_result, sizeof(double) * 4 * evals, hipHostMallocDefault));
+    L.cap_ws = ws; L.cap_blocks = blocks; L.cap_evals = evals;
+    return BV_OK;
+}
+
+extern "C" int bv_server_create(int device, int nlanes, double ws_bytes_per_lane,
+                                int max_evals_per_batch, bv_server **out) {
+    if (!out || nlanes < 1 || nlanes > 16 || !(ws_bytes_per_lane > 0) || max_evals_per_batch < 1)
+        return BV_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev)
+        return BV_ERR_NO_DEVICE;
+    BV_HIP(hipSetDevice(device));
+    bv_server *s = new bv_server();
+    s->device = device;
+    s->ws_budget = ws_bytes_per_lane;
+    s->max_evals = max_evals_per_batch;
+    s->lanes.resize(nlanes);
+    for (bv_lane &L : s->lanes) {
+        if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess) {
+            bv_server_destroy(s);
+            return BV_ERR_HIP;
+        }
+    }
+    *out = s;
+    return BV_OK;
+}
+
+extern "C" int bv_server_attach(bv_server *s, bv_ctx *dataset, int *dataset_id) {
+    if (!s || !dataset || !dataset_id) return BV_ERR_ARG;
+    if (dataset->device != s->device) return BV_ERR_ARG;
+    s->datasets.push_back(dataset);
+    *dataset_id = (int)s->datasets.size() - 1;
+    return BV_OK;
+}
+
+extern "C" int bv_server_submit(bv_server *s, int dataset_id, int nparam, const double *theta,
+                                int64_t tag) {
+    if (!s || !theta || dataset_id < 0 || dataset_id >= (int)s->datasets.size()) return BV_ERR_ARG;
+    const bv_ctx *c = s->datasets[dataset_id];
+    if (nparam != bv_nparams(c->kernel)) return BV_ERR_ARG;
+    bv_request r;
+    memset(&r, 0, sizeof(r));
+    r.dataset = dataset_id;
+    r.tag = tag;
+    for (int i = 0; i < nparam; ++i) r.theta[i] = theta[i];
+    /* reject bad parameters now, so a batch never fails for one request */
+    bv_kparams kp;
+    int kind;
+    if (bv_make_kparams(c, r.theta, &kp, &kind) != BV_OK) return BV_ERR_ARG;
+    if (s->qhead > 0 && (s->qhead == s->queue.size() || s->qhead > 4096)) {
+        s->queue.erase(s->queue.begin(), s->queue.begin() + (ptrdiff_t)s->qhead);
+        s->qhead = 0;
+    }
+    s->queue.push_back(r);
+    return BV_OK;
+}
+
+/* Admit queued requests into the idle lane L and launch them. */
+static int bv_server_launch(bv_server *s, bv_lane &L) {
+    L.reqs.clear();
+    int64_t ws = 0, blocks = 0;
+    while (s->qhead < s->queue.size() && (int)L.reqs.size() < s->max_evals) {
+        const bv_request &r = s->queue[s->qhead];
+        const bv_ctx *c = s->datasets[r.dataset];
+        const double bytes = 8.0 * (double)(ws + c->ws_per_eval);
+        if (!L.reqs.empty() && bytes > s->ws_budget) break; /* admission by workspace bytes */
+        ws += c->ws_per_eval;
+        blocks += c->B;
+        L.reqs.push_back(r);
+        ++s->qhead;
+    }
+    const int k = (int)L.reqs.size();
+    if (k == 0) return BV_OK;
+    int rc;
+    if ((rc = bv_lane_reserve(L, ws, blocks, k))) return rc;
+    /* slots, then descriptors grouped by kind, each group sorted by padded size */
+    int64_t ws_off = 0, blk_off = 0;
+    int kind_of[8] = {0};
+    for (int e = 0; e < k; ++e) {
+        const bv_request &r = L.reqs[e];
+        const bv_ctx *c = s->datasets[r.dataset];
+        bv_eval_slot &sl = L.h_slots[e];
+        memset(&sl, 0, sizeof(sl));
+        if ((rc = bv_make_kparams(c, r.theta, &sl.kp, &sl.kind))) return rc;
+        sl.x = c->d_x; sl.y = c->d_y; sl.z = c->d_z; sl.obs = c->d_obs;
+        sl.nbr = c->d_nbr; sl.cx = c->d_cx; sl.cy = c->d_cy; sl.cz = c->d_cz; sl.cobs = c->d_cobs;
+        sl.cs = c->cs;
+        sl.nblocks = c->B;
+        sl.batch_num = c->d_batch_num;
+        sl.logdet = L.d_logdet + blk_off;
+        sl.dot = L.d_dot + blk_off;
+        sl.info = L.d_info + blk_off;
+        sl.result = L.d_result + 4 * e;
+        sl.log2pi = log(2 * 3.141592653589793);
+        kind_of[sl.kind] += c->B;
+        blk_off += c->B;
+    }
+    /* descriptor ranges per kind */
+    int64_t start[8], fill[8], kmax_mp[8];
+    int64_t acc = 0;
+    for (int kd = 0; kd < 8; ++kd) { start[kd] = acc; fill[kd] = acc; acc += kind_of[kd]; kmax_mp[kd] = 0; }
+    ws_off = 0;
+    for (int e = 0; e < k; ++e) {
+        const bv_ctx *c = s->datasets[L.reqs[e].dataset];
+        const int kd = L.h_slots[e].kind;
+        for (const bv_block_desc &d0 : c->base_desc) {
+            bv_block_desc d = d0;
+            d.eval = e;
+            d.ws_off = d0.ws_off + ws_off;
+            L.h_desc[fill[kd]++] = d;
+            kmax_mp[kd] = std::max<int64_t>(kmax_mp[kd], d0.mp);
+        }
+        ws_off += c->ws_per_eval;
+    }
+    for (int kd = 0; kd < 8; ++kd)
+        std::stable_sort(L.h_desc + start[kd], L.h_desc + fill[kd],
+                         [](const bv_block_desc &a, const bv_block_desc &b) { return a.mp > b.mp; });
+    BV_HIP(hipSetDevice(s->device));
+    BV_HIP(hipMemcpyAsync(L.d_slots, L.h_slots, sizeof(bv_eval_slot) * k, hipMemcpyHostToDevice,
+                          L.stream));
+    BV_HIP(hipMemcpyAsync(L.d_desc, L.h_desc, sizeof(bv_block_desc) * (size_t)blocks,
+                          hipMemcpyHostToDevice, L.stream));
+    for (int kd = 0; kd < 8; ++kd) {
+        const unsigned nblk = (unsigned)kind_of[kd];
+        if (!nblk) continue;
+        const bv_block_desc *dd = L.d_desc + start[kd];
+        const size_t lds = sizeof(double) * 4 * (size_t)kmax_mp[kd];
+        if (kd == BV_KIND_GENERAL || kd == BV_KIND_SPACETIME) {
+            const unsigned T = (unsigned)(kmax_mp[kd] / BV_TILE);
+            BV_HIP(bv_launch_gen(kd, T * (T + 1) / 2, nblk, L.stream, dd, L.d_slots, L.d_ws));
+        }
+        BV_HIP(bv_launch_block(kd, nblk, lds, L.stream, dd, L.d_slots, L.d_ws));
+    }
+    BV_HIP(bv_launch_reduce((unsigned)k, L.stream, L.d_slots));
+    BV_HIP(hipMemcpyAsync(L.h_result, L.d_result, sizeof(double) * 4 * k, hipMemcpyDeviceToHost,
+                          L.stream));
+    BV_HIP(hipEventRecord(L.done, L.stream));
+    L.busy = true;
+    return BV_OK;
+}
+
+extern "C" int bv_server_poll(bv_server *s, int wait, int max_out, int64_t *tags, double *llh,
+                              int *status) {
+    if (!s || max_out < 0 || (max_out > 0 && (!tags || !llh))) return BV_ERR_ARG;
+    BV_HIP(hipSetDevice(s->device));
+    int nout = 0;
+    for (;;) {
+        /* collect finished lanes (only whole batches, while there is room) */
+        for (bv_lane &L : s->lanes) {
+            if (!L.busy || nout + (int)L.reqs.size() > max_out) continue;
+            const hipError_t q = hipEventQuery(L.done);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) return BV_ERR_HIP;
+            for (size_t e = 0; e < L.reqs.size(); ++e) {
+                const int bad = (int)L.h_result[4 * e + 1];
+                tags[nout] = L.reqs[e].tag;
+                llh[nout] = bad >= 0 ? -INFINITY : L.h_result[4 * e];
+                if (status) status[nout] = bad >= 0 ? BV_ERR_NOT_PD : BV_OK;
+                ++nout;
+            }
+            L.busy = false;
+            L.reqs.clear();
+        }
+        /* start queued work on idle lanes */
+        for (bv_lane &L : s->lanes) {
+            if (L.busy || s->qhead >= s->queue.size()) continue;
+            const int rc = bv_server_launch(s, L);
+            if (rc != BV_OK) return rc;
+        }
+        if (nout > 0 || !wait) return nout;
+        /* wait for the oldest busy lane that fits in the output */
+        bv_lane *wl = nullptr;
+        for (bv_lane &L : s->lanes)
+            if (L.busy && nout + (int)L.reqs.size() <= max_out) { wl = &L; break; }
+        if (!wl) return nout; /* nothing in flight (or no room to return it) */
+        BV_HIP(hipEventSynchronize(wl->done));
+    }
+}
+
+extern "C" int bv_server_pending(bv_server *s) {
+    if (!s) return BV_ERR_ARG;
+    int n = (int)(s->queue.size() - s->qhead);
+    for (const bv_lane &L : s->lanes)
+        if (L.busy) n += (int)L.reqs.size();
+    return n;
+}
+
+extern "C" void bv_server_destroy(bv_server *s) {
+    if (!s) return;
+    hipSetDevice(s->device);
+    for (bv_lane &L : s->lanes) {
+        if (L.stream) hipStreamSynchronize(L.stream);
+        bv_lane_free(L);
+        if (L.done) hipEventDestroy(L.done);
+        if (L.stream) hipStreamDestroy(L.stream);
+    }
+    delete s;
+}

@@ -1,0 +1,155 @@
+"""Serving benchmark: many optimizer sessions on several resident datasets.
+
+Each session is bound to one dataset and submits its next parameter vector
+as soon as its previous result returns (closed loop, like an optimizer
+waiting for its objective).  The server merges whatever is queued into shared
+launches on several lanes.  The plain form evaluates the same request
+sequence one evaluation at a time with bv_llh.  Reported: evaluations/s of
+both, the served p50/p99 latency from submit to result, the plain per-call
+p50/p99, and the largest relative difference between a served result and the
+same evaluation alone.
+"""
+import time
+
+import numpy as np
+
+import blockvecchia_amd as bv
+from blockvecchia_amd import datagen
+
+# (n, cs, smoothness nu): block count n/13, as the reference's perf shapes.
+# Many small datasets: single evaluations leave most of the GPU idle.
+SMALL_DATASETS = [
+    (1000, 30, 0.5), (1000, 60, 1.5), (2000, 30, 0.5), (2000, 60, 2.5),
+    (3000, 60, 0.5), (3000, 30, 1.5), (5000, 60, 0.5), (5000, 30, 0.5),
+]
+# Mixed sizes up to the n = 20k / cs = 300 shape: the large evaluations
+# already fill the GPU on their own.
+DEFAULT_DATASETS = [
+    (2000, 30, 0.5), (2000, 60, 1.5), (5000, 60, 0.5), (5000, 150, 2.5),
+    (10000, 150, 0.5), (10000, 60, 1.5), (20000, 300, 0.5), (20000, 150, 0.5),
+]
+
+
+def build_datasets(specs=DEFAULT_DATASETS, seed=0):
+    engines, truths = [], []
+    for i, (n, cs, nu) in enumerate(specs):
+        x, y = datagen.uniform_locations(n, seed + i)
+        obs = np.random.default_rng(seed + i).standard_normal(n)
+        blocks = bv.partition(x, y, obs, max(1, n // 13), cs, seed=0)
+        engines.append(bv.Engine(blocks))
+        truths.append(np.array([1.5, 0.1, nu]))
+    return engines, truths
+
+
+def make_requests(truths, sessions, per_session, seed=1):
+    """Per session: dataset index and its sequence of parameter vectors
+    (the true theta with a few percent jitter on variance and range)."""
+    rng = np.random.default_rng(seed)
+    plan = []
+    for s in range(sessions):
+        d = s % len(truths)
+        ths = []
+        for _ in range(per_session):
+            th = truths[d].copy()
+            th[0] *= 1.0 + 0.05 * rng.standard_normal()
+            th[1] *= 1.0 + 0.05 * rng.standard_normal()
+            ths.append(th)
+        plan.append((d, ths))
+    return plan
+
+
+def run_plain(engines, plan):
+    """One evaluation at a time, sessions interleaved round-robin."""
+    order = []
+    per = len(plan[0][1])
+    for k in range(per):
+        for s, (d, ths) in enumerate(plan):
+            order.append((s, k, d, ths[k]))
+    lat = []
+    out = {}
+    t0 = time.perf_counter()
+    for s, k, d, th in order:
+        a = time.perf_counter()
+        out[(s, k)] = engines[d].llh(th)
+        lat.append(time.perf_counter() - a)
+    wall = time.perf_counter() - t0
+    return out, wall, np.array(lat)
+
+
+def run_served(engines, plan, lanes=4, ws_bytes=2 << 30, max_evals=64):
+    srv = bv.Server(lanes=lanes, ws_bytes_per_lane=ws_bytes, max_evals_per_batch=max_evals)
+    ids = [srv.attach(e) for e in engines]
+    nxt = [0] * len(plan)
+    t_sub = {}
+    out = {}
+    lat = []
+
+    def submit(s):
+        d, ths = plan[s]
+        k = nxt[s]
+        tag = s * 1_000_000 + k
+        t_sub[tag] = time.perf_counter()
+        srv.submit(ids[d], ths[k], tag)
+        nxt[s] += 1
+
+    t0 = time.perf_counter()
+    for s in range(len(plan)):
+        submit(s)
+    total = sum(len(ths) for _, ths in plan)
+    done = 0
+    while done < total:
+        res = srv.poll(wait=True)
+        now = time.perf_counter()
+        for tag, v, st in res:
+            s, k = divmod(tag, 1_000_000)
+            out[(s, k)] = v
+            lat.append(now - t_sub.pop(tag))
+            done += 1
+            if nxt[s] < len(plan[s][1]):
+                submit(s)
+    wall = time.perf_counter() - t0
+    srv.close()
+    return out, wall, np.array(lat)
+
+
+def benchmark(sessions=32, per_session=16, lanes=4, ws_bytes=2 << 30, max_evals=64,
+              specs=DEFAULT_DATASETS):
+    engines, truths = build_datasets(specs)
+    plan = make_requests(truths, sessions, per_session)
+    # warm-up: every dataset once, both paths
+    for e, t in zip(engines, truths):
+        e.llh(t)
+    run_served(engines, [(d, [t]) for d, t in enumerate(truths)], lanes, ws_bytes, max_evals)
+    plain, t_plain, lat_plain = run_plain(engines, plan)
+    served, t_served, lat_served = run_served(engines, plan, lanes, ws_bytes, max_evals)
+    worst = max(abs(served[k] - plain[k]) / abs(plain[k]) for k in plain)
+    nreq = len(plain)
+    for e in engines:
+        e.close()
+    return {
+        "requests": nreq,
+        "sessions": sessions,
+        "datasets": [{"n": n, "cs": cs, "nu": nu, "bc": max(1, n // 13)} for n, cs, nu in specs],
+        "lanes": lanes,
+        "evals_per_s": nreq / t_served,
+        "plain_evals_per_s": nreq / t_plain,
+        "speedup_vs_plain": t_plain / t_served,
+        "latency_ms": {"p50": float(np.percentile(lat_served, 50) * 1e3),
+                       "p99": float(np.percentile(lat_served, 99) * 1e3)},
+        "plain_latency_ms": {"p50": float(np.percentile(lat_plain, 50) * 1e3),
+                             "p99": float(np.percentile(lat_plain, 99) * 1e3)},
+        "max_rel_diff_vs_alone": worst,
+    }
+
+
+def scenarios():
+    """The two serving scenarios reported by bench.py."""
+    return {
+        "small_datasets": benchmark(sessions=64, per_session=16, specs=SMALL_DATASETS),
+        "mixed_sizes": benchmark(sessions=32, per_session=16, specs=DEFAULT_DATASETS),
+    }
+
+
+if __name__ == "__main__":
+    import json
+    print(json.dumps(scenarios()))

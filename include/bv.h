@@ -114,6 +114,43 @@ int bv_iterations(bv_ctx *ctx);
 
 void bv_destroy(bv_ctx *ctx);
 
+/* ---- request server ---------------------------------------------------
+ * No counterpart in the reference, which evaluates one theta at a time for
+ * one NLopt session.  For callers that serve many optimizer sessions on
+ * different datasets: each dataset is a bv_ctx from bv_create (resident in
+ * HBM); requests queue in arrival order; bv_server_poll admits them into idle
+ * lanes (one stream each) while their workspace bytes fit the lane budget,
+ * merges their blocks into shared launches, and returns finished results.
+ * Calls on one server are not thread-safe; a result does not depend on the
+ * batch it ran in. */
+typedef struct bv_server bv_server;
+
+/* nlanes concurrent batches (1..16), each with ws_bytes_per_lane of
+ * workspace (a larger single request still runs, alone) and at most
+ * max_evals_per_batch evaluations. */
+int bv_server_create(int device, int nlanes, double ws_bytes_per_lane,
+                     int max_evals_per_batch, bv_server **out);
+
+/* Register a dataset (the context stays owned by the caller and must outlive
+ * the server's use of it). */
+int bv_server_attach(bv_server *s, bv_ctx *dataset, int *dataset_id);
+
+/* Queue one evaluation; tag is returned with its result. */
+int bv_server_submit(bv_server *s, int dataset_id, int nparam, const double *theta,
+                     int64_t tag);
+
+/* Collect finished evaluations (up to max_out: tags, llh, status) and start
+ * queued ones on idle lanes.  wait != 0 blocks until at least one result is
+ * ready (or nothing is queued or running).  Returns the number of results,
+ * or a negative status. */
+int bv_server_poll(bv_server *s, int wait, int max_out, int64_t *tags, double *llh,
+                   int *status);
+
+/* Requests queued or running. */
+int bv_server_pending(bv_server *s);
+
+void bv_server_destroy(bv_server *s);
+
 /* Dense Matern block A[i + j*m] = K(l1[i], l2[j]) computed on the device with
  * the engine's covariance code (core_dcmg, src/ckernel.cpp:1057-1093).
  * Host arrays in, host array out.  Used for entrywise parity tests. */

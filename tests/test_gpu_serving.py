@@ -1,0 +1,105 @@
+"""Request server: evaluations from several datasets and covariance kinds,
+merged into shared launches on two lanes, give the same log-likelihood as each
+evaluation alone (bv_llh) and as the oracle, at the stated 1e-10 relative
+tolerance; requests submitted while batches are on the GPU are admitted and
+complete.  Runs on the GPU through the C ABI."""
+import sys
+
+import numpy as np
+import pytest
+
+import blockvecchia_amd as bv
+from blockvecchia_amd import datagen
+import oracle
+
+from conftest import LLH_RTOL
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[serving] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def _datasets():
+    specs = [(600, 40, 20, 1), (900, 60, 30, 2), (1200, 80, 60, 3), (1500, 100, 25, 4)]
+    out = []
+    for n, bc, cs, seed in specs:
+        x, y = datagen.uniform_locations(n, seed)
+        # observations drawn from the model (dense FP64 Cholesky on the CPU)
+        obs = datagen.sample_dense(x, y, 1.5, 0.1, 0.5, seed=seed)
+        out.append(bv.partition(x, y, obs, bc, cs, seed=0))
+    return out
+
+
+def _requests(nds):
+    thetas = [[1.5, 0.1, 0.5], [1.0, 0.08, 1.5], [1.2, 0.05, 2.5], [1.0, 0.07, 0.8]]
+    reqs = []
+    tag = 0
+    for rep in range(3):
+        for d in range(nds):
+            th = list(thetas[(d + rep) % len(thetas)])
+            th[0] *= 1.0 + 0.05 * rep
+            reqs.append((tag, d, th))
+            tag += 1
+    return reqs
+
+
+def test_server_matches_alone_and_oracle():
+    blocks = _datasets()
+    engines = [bv.Engine(b) for b in blocks]
+    reqs = _requests(len(blocks))
+    _say("alone: %d requests" % len(reqs))
+    alone = {tag: engines[d].llh(th) for tag, d, th in reqs}
+    # a small workspace budget forces several batches per lane
+    srv = bv.Server(lanes=2, ws_bytes_per_lane=3e6, max_evals_per_batch=5)
+    ids = [srv.attach(e) for e in engines]
+    got = {}
+    _say("mix: submit half, poll, submit the rest while batches run")
+    half = len(reqs) // 2
+    for tag, d, th in reqs[:half]:
+        srv.submit(ids[d], th, tag)
+    for t, v, st in srv.poll(wait=False):
+        got[t] = (v, st)
+    for tag, d, th in reqs[half:]:
+        srv.submit(ids[d], th, tag)
+        for t, v, st in srv.poll(wait=False):
+            got[t] = (v, st)
+    while srv.pending() > 0:
+        for t, v, st in srv.poll(wait=True):
+            got[t] = (v, st)
+    assert sorted(got) == sorted(alone)
+    for tag, d, th in reqs:
+        v, st = got[tag]
+        assert st == bv.BV_OK
+        assert abs(v - alone[tag]) <= LLH_RTOL * abs(alone[tag]), (tag, v, alone[tag])
+    _say("oracle check (the well-conditioned requests: nu <= 1.5)")
+    for tag, d, th in reqs:
+        if th[2] > 1.5:
+            continue  # smooth and misspecified: checked alone vs mix only
+        ref = oracle.llh(blocks[d], th)
+        assert abs(got[tag][0] - ref) <= LLH_RTOL * abs(ref), (tag, got[tag][0], ref)
+    srv.close()
+    for e in engines:
+        e.close()
+
+
+def test_server_large_request_runs_alone_and_rejects_bad_theta():
+    blocks = _datasets()[:2]
+    engines = [bv.Engine(b) for b in blocks]
+    srv = bv.Server(lanes=1, ws_bytes_per_lane=1.0, max_evals_per_batch=4)
+    ids = [srv.attach(e) for e in engines]
+    with pytest.raises(RuntimeError):
+        srv.submit(ids[0], [1.0, -0.1, 0.5], 99)
+    srv.submit(ids[0], [1.5, 0.1, 0.5], 1)
+    srv.submit(ids[1], [1.5, 0.1, 0.5], 2)
+    got = {}
+    while srv.pending() > 0:
+        for t, v, st in srv.poll(wait=True):
+            got[t] = v
+    assert got[1] == engines[0].llh([1.5, 0.1, 0.5])
+    assert got[2] == engines[1].llh([1.5, 0.1, 0.5])
+    srv.close()
+    for e in engines:
+        e.close()
