@@ -19,6 +19,10 @@ def _ensure_built():
     if not os.path.exists(lib):
         from blockvecchia_amd import build
         build.build_lib()
+    drv = os.path.join(ROOT, "blockvecchia_amd", "bin", "test_dvecchia_batch")
+    if not os.path.exists(drv):
+        from blockvecchia_amd import build
+        build.build_driver()
     if not os.path.exists(orc):
         from oracle import build as obuild
         obuild.build_oracle()
