@@ -27,6 +27,13 @@ hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t s
 hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,
                          const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
 hipError_t bv_launch_reduce(unsigned nevals, hipStream_t st, const bv_eval_slot *slots);
+hipError_t bv_launch_step(int kind, unsigned nitems, int j, hipStream_t st,
+                          const bv_step_item *items, const bv_block_desc *desc,
+                          const bv_eval_slot *slots, double *ws, double *part_logdet,
+                          double *part_dot, int32_t *part_info, int pmax);
+hipError_t bv_launch_step_parts(int ndesc, hipStream_t st, const bv_block_desc *desc,
+                                const bv_eval_slot *slots, const double *part_logdet,
+                                const double *part_dot, const int32_t *part_info, int pmax);
 hipError_t bv_launch_matern(int m, int n, const double *x1, const double *y1, const double *z1,
                             const double *x2, const double *y2, const double *z2,
                             const bv_kparams &kp, int kind, double *A);
@@ -75,6 +82,16 @@ struct bv_ctx {
     double last_ms = 0.0;
     int iterations = 0;
     int variant = 0; /* factorization kernel: 0 = four wavefronts per block, 1 = one */
+    int stepped = 0; /* 1: one launch per panel step over all blocks */
+    /* stepped factorization: work items of every step for desc_evals evaluations */
+    bv_step_item *d_items = nullptr;
+    std::vector<int64_t> step_off;          /* [chunk][pmax + 1] item offsets      */
+    int nchunks = 1, nstreams = 1;          /* block chunks, and streams they cycle over */
+    hipStream_t cstream[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t cev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int pmax = 0;
+    double *d_part_logdet = nullptr, *d_part_dot = nullptr;
+    int32_t *d_part_info = nullptr;
 };
 
 static int bv_nparams(int kernel) {
@@ -99,6 +116,12 @@ static void bv_free_eval_buffers(bv_ctx *c) {
     if (c->d_info) hipFree(c->d_info);
     if (c->d_result) hipFree(c->d_result);
     if (c->h_result) hipHostFree(c->h_result);
+    if (c->d_items) hipFree(c->d_items);
+    if (c->d_part_logdet) hipFree(c->d_part_logdet);
+    if (c->d_part_dot) hipFree(c->d_part_dot);
+    if (c->d_part_info) hipFree(c->d_part_info);
+    c->d_items = nullptr; c->d_part_logdet = nullptr; c->d_part_dot = nullptr;
+    c->d_part_info = nullptr;
     c->d_desc = nullptr; c->d_slots = nullptr; c->h_slots = nullptr; c->d_ws = nullptr;
     c->d_logdet = nullptr; c->d_dot = nullptr; c->d_info = nullptr;
     c->d_result = nullptr; c->h_result = nullptr;
@@ -119,6 +142,10 @@ static int bv_reserve(bv_ctx *c, int k) {
     BV_HIP(hipMalloc((void **)&c->d_info, sizeof(int32_t) * B * k));
     BV_HIP(hipMalloc((void **)&c->d_result, sizeof(double) * 4 * k));
     BV_HIP(hipHostMalloc((void **)&c->h_result, sizeof(double) * 4 * k, hipHostMallocDefault));
+    c->pmax = c->max_mp / BV_NB;
+    BV_HIP(hipMalloc((void **)&c->d_part_logdet, sizeof(double) * B * k * c->pmax));
+    BV_HIP(hipMalloc((void **)&c->d_part_dot, sizeof(double) * B * k * c->pmax));
+    BV_HIP(hipMalloc((void **)&c->d_part_info, sizeof(int32_t) * B * k * c->pmax));
     c->cap_evals = k;
     return BV_OK;
 }
@@ -138,6 +165,58 @@ static int bv_layout(bv_ctx *c, int k) {
         }
     BV_HIP(hipMemcpyAsync(c->d_desc, all.data(), sizeof(bv_block_desc) * all.size(),
                           hipMemcpyHostToDevice, c->stream));
+    /* stepped factorization: per step j, for every descriptor with more than
+       j panels, its rows 2j..T-1 in items of BV_STEP_ROWS row tiles (the item
+       holding the diagonal block first).  Workgroups are dealt to the XCDs
+       round-robin (workgroup w runs on XCD w % 8), so the items of one
+       descriptor get ids of one residue mod 8: they share the panel-row and
+       W tiles through that XCD's L2.  Descriptors go to the least loaded XCD;
+       shorter queues are padded with empty items. */
+    const int NX = 8;
+    std::vector<bv_step_item> items;
+    const int NC = c->nchunks;
+    c->step_off.assign((size_t)NC * (c->pmax + 1), 0);
+    for (int ch = 0; ch < NC; ++ch) {
+        for (int j = 0; j < c->pmax; ++j) {
+            c->step_off[(size_t)ch * (c->pmax + 1) + j] = (int64_t)items.size();
+            std::vector<std::vector<bv_step_item>> q(NX);
+            for (size_t i = (size_t)ch; i < all.size(); i += (size_t)NC) {
+                const int T = all[i].mp / BV_TILE, P = all[i].mp / BV_NB;
+                if (j >= P) continue;
+                size_t x = 0;
+                for (int t = 1; t < NX; ++t)
+                    if (q[t].size() < q[x].size()) x = t;
+                for (int r = 2 * j; r < T; r += BV_STEP_ROWS) {
+                    bv_step_item it;
+                    it.desc = (int32_t)i;
+                    it.row0 = r;
+                    it.nrows = std::min(BV_STEP_ROWS, T - r);
+                    it.pad_ = 0;
+                    q[x].push_back(it);
+                }
+            }
+            size_t len = 0;
+            for (int t = 0; t < NX; ++t) len = std::max(len, q[t].size());
+            for (size_t w = 0; w < len * NX; ++w) {
+                const std::vector<bv_step_item> &qq = q[w % NX];
+                const size_t k2 = w / NX;
+                if (k2 < qq.size()) items.push_back(qq[k2]);
+                else {
+                    bv_step_item e;
+                    e.desc = -1; e.row0 = 0; e.nrows = 0; e.pad_ = 0;
+                    items.push_back(e);
+                }
+            }
+        }
+        c->step_off[(size_t)ch * (c->pmax + 1) + c->pmax] = (int64_t)items.size();
+    }
+    if (c->d_items) hipFree(c->d_items);
+    c->d_items = nullptr;
+    if (!items.empty()) {
+        BV_HIP(hipMalloc((void **)&c->d_items, sizeof(bv_step_item) * items.size()));
+        BV_HIP(hipMemcpyAsync(c->d_items, items.data(), sizeof(bv_step_item) * items.size(),
+                              hipMemcpyHostToDevice, c->stream));
+    }
     BV_HIP(hipStreamSynchronize(c->stream));
     c->desc_evals = k;
     return BV_OK;
@@ -205,6 +284,9 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     c->verbose = p->verbose;
     /* factorization kernel: BV_FACTOR_KERNEL=1w selects the one-wavefront one */
     if (const char *v = getenv("BV_FACTOR_KERNEL")) c->variant = (strcmp(v, "1w") == 0) ? 1 : 0;
+    if (const char *v = getenv("BV_PATH")) c->stepped = (strcmp(v, "stepped") == 0) ? 1 : 0;
+    if (const char *v = getenv("BV_CHUNKS")) c->nchunks = std::max(1, std::min(64, atoi(v)));
+    if (const char *v = getenv("BV_CHUNK_STREAMS")) c->nstreams = std::max(1, std::min(4, atoi(v)));
     c->h_batch_num.assign(p->batch_num, p->batch_num + p->batch_count);
     c->h_accum.assign((size_t)c->B + 1, 0);
     for (int i = 0; i < c->B; ++i) {
@@ -243,6 +325,18 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         bv_destroy(c);
         return BV_ERR_HIP;
+    }
+    if (c->stepped && c->nstreams > 1) {
+        for (int i = 0; i < c->nstreams; ++i)
+            if (hipStreamCreateWithFlags(&c->cstream[i], hipStreamNonBlocking) != hipSuccess) {
+                bv_destroy(c);
+                return BV_ERR_HIP;
+            }
+        for (int i = 0; i <= c->nstreams; ++i)
+            if (hipEventCreateWithFlags(&c->cev[i], hipEventDisableTiming) != hipSuccess) {
+                bv_destroy(c);
+                return BV_ERR_HIP;
+            }
     }
     int rc = BV_OK;
     const size_t n = (size_t)c->n;
@@ -321,7 +415,36 @@ static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *ou
             BV_HIP(bv_launch_gen(kind, T * (T + 1) / 2, nblk, c->stream, c->d_desc, c->d_slots,
                                  c->d_ws));
         }
-        if (c->variant == 1)
+        if (c->stepped) {
+            /* chunks of blocks cycle over the side streams: a few chunks are
+               active at a time, so their workspace stays in the last-level
+               cache while their steps interleave on the device */
+            const bool side = c->nstreams > 1;
+            if (side) {
+                BV_HIP(hipEventRecord(c->cev[c->nstreams], c->stream));
+                for (int i = 0; i < c->nstreams; ++i)
+                    BV_HIP(hipStreamWaitEvent(c->cstream[i], c->cev[c->nstreams], 0));
+            }
+            for (int ch = 0; ch < c->nchunks; ++ch) {
+                hipStream_t st = side ? c->cstream[ch % c->nstreams] : c->stream;
+                const int64_t *off = c->step_off.data() + (size_t)ch * (c->pmax + 1);
+                for (int j = 0; j < c->pmax; ++j) {
+                    const int64_t o = off[j], cnt = off[j + 1] - o;
+                    BV_HIP(bv_launch_step(kind, (unsigned)cnt, j, st, c->d_items + o, c->d_desc,
+                                          c->d_slots, c->d_ws, c->d_part_logdet, c->d_part_dot,
+                                          c->d_part_info, c->pmax));
+                }
+            }
+            if (side) {
+                for (int i = 0; i < c->nstreams; ++i) {
+                    BV_HIP(hipEventRecord(c->cev[i], c->cstream[i]));
+                    BV_HIP(hipStreamWaitEvent(c->stream, c->cev[i], 0));
+                }
+            }
+            BV_HIP(bv_launch_step_parts((int)nblk, c->stream, c->d_desc, c->d_slots,
+                                        c->d_part_logdet, c->d_part_dot, c->d_part_info,
+                                        c->pmax));
+        } else if (c->variant == 1)
             BV_HIP(bv_launch_block_1w(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
         else
             BV_HIP(bv_launch_block(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
@@ -415,6 +538,10 @@ extern "C" void bv_destroy(bv_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 4; ++i)
+        if (c->cstream[i]) { hipStreamSynchronize(c->cstream[i]); hipStreamDestroy(c->cstream[i]); }
+    for (int i = 0; i < 5; ++i)
+        if (c->cev[i]) hipEventDestroy(c->cev[i]);
     bv_free_eval_buffers(c);
     double *bufs[] = {c->d_x, c->d_y, c->d_z, c->d_obs, c->d_cx, c->d_cy, c->d_cz, c->d_cobs};
     for (double *b : bufs)

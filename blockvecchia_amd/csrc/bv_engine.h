@@ -44,6 +44,18 @@ struct bv_block_desc {
     int64_t obs_off;   /* offset of the block's first point (batchNumAccum) */
 };
 
+/* One work item of a panel step of the stepped factorization: BV_STEP_ROWS
+ * consecutive row tiles of one block (descriptor); desc < 0 marks padding. */
+#ifndef BV_STEP_ROWS
+#define BV_STEP_ROWS 2
+#endif
+struct bv_step_item {
+    int32_t desc;      /* index into the descriptor array, or -1          */
+    int32_t row0;      /* first row tile (2j for the item holding the diagonal block) */
+    int32_t nrows;     /* row tiles in this item, 1..BV_STEP_ROWS          */
+    int32_t pad_;
+};
+
 /* Everything one evaluation needs: dataset pointers, covariance parameters
  * and where its per-block results go. */
 struct bv_eval_slot {

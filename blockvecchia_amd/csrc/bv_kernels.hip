@@ -85,9 +85,9 @@ __device__ __forceinline__ void bv_point(const bv_block_desc &d, const bv_eval_s
    the panel operand is negated, so the products subtract).  C is a compile-time
    count so the loop is straight-line matrix-core code; the loads of tile
    column kt+1 are in flight while the products of tile column kt issue. */
-template <int C>
+template <int C, int N>
 __device__ __forceinline__ void bv_gemm_step(const bv_d4 &j0, const bv_d4 &j1, const bv_d4 (&r)[C],
-                                             bv_d4 (&acc)[BV_NRT][2]) {
+                                             bv_d4 (&acc)[N][2]) {
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
 #pragma unroll
@@ -129,11 +129,11 @@ __device__ __forceinline__ void bv_panel_gemm(const double *__restrict__ ws, int
         /* column kt+1 (rt0 is even, so it always exists here) */
         const double *colb = colp + ((int64_t)(T - kt - 1) << 8);
         bv_gemm_load<C>(colb, rt0, rbase, rstride, lo, jb0, jb1, rb);
-        bv_gemm_step<C>(ja0, ja1, ra, acc);
+        bv_gemm_step<C, BV_NRT>(ja0, ja1, ra, acc);
         /* column kt+2, or column kt+1 again after the last pair */
         const double *cola = (kt + 2 < rt0) ? colb + ((int64_t)(T - kt - 2) << 8) : colb;
         bv_gemm_load<C>(cola, rt0, rbase, rstride, lo, ja0, ja1, ra);
-        bv_gemm_step<C>(jb0, jb1, rb, acc);
+        bv_gemm_step<C, BV_NRT>(jb0, jb1, rb, acc);
         colp = cola;
     }
 }
@@ -377,15 +377,15 @@ __device__ __forceinline__ void bv_panel_logdet(int lane, int J0, int m, int cs0
    16x16:  L11, W11 = sweep(A11);  L21 = A21 W11^T;  A22' = A22 - L21 L21^T;
    L22, W22 = sweep(A22');  W21 = -W22 (L21 W11).  W = L_jj^-1 goes to sW as
    the tiles (0,0), (1,0), (1,1) in accumulator layout; pivots to sPiv. */
-__device__ __forceinline__ void bv_diag_factor_wave(const double *__restrict__ ws, int T, int rt0,
-                                                    int lane, int J0, int m, int cs0,
-                                                    double *sCol, double *sRow, double *sT,
-                                                    double *sW, double *sPiv, double *s_dotd,
-                                                    int *s_info) {
+__device__ __forceinline__ void bv_diag_factor_tiles(const double *t11, const double *t21,
+                                                     const double *t22, int lane, int J0, int m,
+                                                     int cs0, double *sCol, double *sRow,
+                                                     double *sT, double *sW, double *sPiv,
+                                                     double *s_dotd, int *s_info) {
     const int lr = lane & 15, lq = lane >> 4;
-    const bv_d4 a11 = *(const bv_d4 *)(ws + bv_tile_off(rt0, rt0, T) + 4 * lane);
-    const bv_d4 a21 = *(const bv_d4 *)(ws + bv_tile_off(rt0 + 1, rt0, T) + 4 * lane);
-    const bv_d4 a22 = *(const bv_d4 *)(ws + bv_tile_off(rt0 + 1, rt0 + 1, T) + 4 * lane);
+    const bv_d4 a11 = *(const bv_d4 *)(t11 + 4 * lane);
+    const bv_d4 a21 = *(const bv_d4 *)(t21 + 4 * lane);
+    const bv_d4 a22 = *(const bv_d4 *)(t22 + 4 * lane);
     double dacc = 0.0;
     int info = 0;
     const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
@@ -413,6 +413,18 @@ __device__ __forceinline__ void bv_diag_factor_wave(const double *__restrict__ w
         *s_dotd += dacc;
         if (info != 0 && *s_info == 0) *s_info = info;
     }
+}
+
+/* The same for the tiles (rt0,rt0), (rt0+1,rt0), (rt0+1,rt0+1) of a workspace
+   with T tile rows. */
+__device__ __forceinline__ void bv_diag_factor_wave(const double *__restrict__ ws, int T, int rt0,
+                                                    int lane, int J0, int m, int cs0,
+                                                    double *sCol, double *sRow, double *sT,
+                                                    double *sW, double *sPiv, double *s_dotd,
+                                                    int *s_info) {
+    bv_diag_factor_tiles(ws + bv_tile_off(rt0, rt0, T), ws + bv_tile_off(rt0 + 1, rt0, T),
+                         ws + bv_tile_off(rt0 + 1, rt0 + 1, T), lane, J0, m, cs0, sCol, sRow,
+                         sT, sW, sPiv, s_dotd, s_info);
 }
 
 template <int KIND>
@@ -589,6 +601,321 @@ bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *_
     }
 }
 
+/* ======================================================================
+ * Stepped factorization: one launch per panel step j over every block.
+ *
+ * A work item is one wavefront on BV_STEP_ROWS consecutive row tiles R of
+ * one block (all rows >= 2j).  At step j it
+ *   (a) solves panel j-1 for its rows, L(R, j-1) = C(R, j-1) W_{j-1}^T, with
+ *       C and W_{j-1} from step j-1, stores L (except rows 2j, 2j+1, whose C
+ *       the other items read in this step and whose L no later step uses)
+ *       and adds row m's squares;
+ *   (b) forms C(R, j) = A(R, j) - sum_k L(R, k) L(2j..2j+1, k)^T: A generated
+ *       in registers (closed forms) or read (LOADA); the last tile-column
+ *       pair k = 2j-2, 2j-1 from registers -- its own rows from (a), the
+ *       panel rows recomputed from C(2j..2j+1, j-1) and W_{j-1} -- and the
+ *       older columns from the workspace;
+ *   (c) stores C(R, j), or, for the item holding rows 2j and 2j+1, factors
+ *       the diagonal block (LDS) and stores W_j over its three tiles, with
+ *       the panel's log-determinant, row-m squares and first failure into
+ *       the per-(block, panel) partial arrays.
+ * Items never wait for each other inside a launch: everything they read was
+ * written by earlier launches.  No barriers; one wavefront per workgroup.
+ * ====================================================================== */
+#ifndef BV_STEP_WAVES_PER_SIMD
+#define BV_STEP_WAVES_PER_SIMD 3
+#endif
+
+/* acc += L(rows, k) L(panel rows, k)^T over tile columns [0, kend) from the
+   workspace, for C consecutive row tiles rbase + s (ping-pong operands). */
+template <int C, int N>
+__device__ __forceinline__ void bv_step_gemm(const double *__restrict__ ws, int T, int j, int kend,
+                                             int rbase, int lane, bv_d4 (&acc)[N][2]) {
+    const int rt0 = 2 * j;
+    if (kend <= 0) return;
+    const int lo = 4 * lane;
+    const double *colp = ws;
+    bv_d4 ja0, ja1, ra[C], jb0, jb1, rb[C];
+    bv_gemm_load<C>(colp, rt0, rbase, 1, lo, ja0, ja1, ra);
+    for (int kt = 0; kt < kend; kt += 2) {
+        const double *colb = colp + ((int64_t)(T - kt - 1) << 8);
+        bv_gemm_load<C>(colb, rt0, rbase, 1, lo, jb0, jb1, rb);
+        bv_gemm_step<C, N>(ja0, ja1, ra, acc);
+        const double *cola = (kt + 2 < kend) ? colb + ((int64_t)(T - kt - 2) << 8) : colb;
+        bv_gemm_load<C>(cola, rt0, rbase, 1, lo, ja0, ja1, ra);
+        bv_gemm_step<C, N>(jb0, jb1, rb, acc);
+        colp = cola;
+    }
+}
+
+/* X = C W^T for one row tile (two column tiles), W = the three tiles of a
+   diagonal block's inverse. */
+__device__ __forceinline__ void bv_solve_pair(const bv_d4 &c0, const bv_d4 &c1, const bv_d4 &w00,
+                                              const bv_d4 &w10, const bv_d4 &w11, bv_d4 &x0,
+                                              bv_d4 &x1) {
+    x0 = (bv_d4){0.0, 0.0, 0.0, 0.0};
+    x1 = (bv_d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        x0 = bv_mfma(w00[g], c0[g], x0);
+        x1 = bv_mfma(w10[g], c0[g], x1);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) x1 = bv_mfma(w11[g], c1[g], x1);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(64, BV_STEP_WAVES_PER_SIMD)
+bv_step_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
+               const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
+               double *ws_base, double *part_logdet, double *part_dot, int32_t *part_info,
+               int pmax) {
+    constexpr int G = BV_STEP_ROWS;
+    __shared__ double sW[3 * 256]; /* diagonal C tiles in, W out */
+    __shared__ double sT[16 * 17];
+    __shared__ double sCol[16], sRow[16], sPiv[BV_NB];
+    __shared__ double s_logdet, s_dotd;
+    __shared__ int s_info;
+    __shared__ double qx[G * BV_TILE + BV_NB], qy[G * BV_TILE + BV_NB], qz[G * BV_TILE + BV_NB],
+        qo[G * BV_TILE + BV_NB];
+    __shared__ bv_kparams s_kp;
+
+    if ((int)blockIdx.x >= nitems) return;
+    const bv_step_item it = items[blockIdx.x];
+    if (it.desc < 0) return;
+    const bv_block_desc d = desc[it.desc];
+    const bv_eval_slot &E = slots[d.eval];
+    const int lane = threadIdx.x;
+    const int m = d.m, cs0 = d.cs0;
+    const int T = d.mp / BV_TILE, P = d.mp / BV_NB;
+    const int rt0 = 2 * j;
+    const int row0 = it.row0, nr = it.nrows;
+    const bool lead = (row0 == rt0);
+    double *ws = ws_base + d.ws_off;
+    const int lr = lane & 15, lq = lane >> 4;
+    double dot_lane = 0.0;
+    bool has_m = false; /* this item holds the row tile of row m */
+#pragma unroll
+    for (int s = 0; s < G; ++s)
+        if (s < nr && (row0 + s) * BV_TILE <= m && m < (row0 + s + 1) * BV_TILE) has_m = true;
+
+    /* (a) solve panel j-1 for the own rows; keep L(R, 2j-2..2j-1) */
+    bv_d4 xl[G][2];
+    bv_d4 jl[2][2]; /* L(2j + r, 2j-2 + c), the panel rows of the last pair */
+    if (j >= 1) {
+        const int p0 = rt0 - 2;
+        const bv_d4 w00 = *(const bv_d4 *)(ws + bv_tile_off(p0, p0, T) + 4 * lane);
+        const bv_d4 w10 = *(const bv_d4 *)(ws + bv_tile_off(p0 + 1, p0, T) + 4 * lane);
+        const bv_d4 w11 = *(const bv_d4 *)(ws + bv_tile_off(p0 + 1, p0 + 1, T) + 4 * lane);
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            if (s < nr) {
+                const int rt = row0 + s;
+                const bv_d4 c0 = *(const bv_d4 *)(ws + bv_tile_off(rt, p0, T) + 4 * lane);
+                const bv_d4 c1 = *(const bv_d4 *)(ws + bv_tile_off(rt, p0 + 1, T) + 4 * lane);
+                bv_solve_pair(c0, c1, w00, w10, w11, xl[s][0], xl[s][1]);
+                /* rows 2j, 2j+1 (the lead's) keep C: other items of this step
+                   read it to recompute the panel rows, and their L is used
+                   only in this step */
+                if (rt >= rt0 + 2) {
+                    *(bv_d4 *)(ws + bv_tile_off(rt, p0, T) + 4 * lane) = xl[s][0];
+                    *(bv_d4 *)(ws + bv_tile_off(rt, p0 + 1, T) + 4 * lane) = xl[s][1];
+                }
+                if (rt * BV_TILE + lr == m) {
+                    const int J0p = p0 * BV_TILE;
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int q0 = J0p + 4 * g + lq, q1 = q0 + BV_TILE;
+                        if (q0 >= cs0 && q0 < m) dot_lane += xl[s][0][g] * xl[s][0][g];
+                        if (q1 >= cs0 && q1 < m) dot_lane += xl[s][1][g] * xl[s][1][g];
+                    }
+                }
+            }
+        }
+        if (j < P) {
+            if (lead) {
+                jl[0][0] = xl[0][0]; jl[0][1] = xl[0][1];
+                jl[1][0] = xl[1][0]; jl[1][1] = xl[1][1];
+            } else {
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const bv_d4 c0 = *(const bv_d4 *)(ws + bv_tile_off(rt0 + r, p0, T) + 4 * lane);
+                    const bv_d4 c1 =
+                        *(const bv_d4 *)(ws + bv_tile_off(rt0 + r, p0 + 1, T) + 4 * lane);
+                    bv_solve_pair(c0, c1, w00, w10, w11, jl[r][0], jl[r][1]);
+                }
+            }
+        }
+        if (has_m) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) dot_lane += __shfl_xor(dot_lane, off, 64);
+            if (lane == 0) part_dot[(int64_t)it.desc * pmax + (j - 1)] = dot_lane;
+        }
+    }
+    if (j >= P) return; /* (only panels < P exist) */
+
+    /* (b) C(R, j): -A, the last column pair from registers, the older
+       columns from the workspace */
+    if (KIND != BV_KIND_LOADA) {
+        /* stage the item's row points and the panel's column points */
+        const int pr = (lane < G * BV_TILE) ? row0 * BV_TILE + lane : -1;
+        if (pr >= 0) {
+            double xv, yv, zv, ov;
+            bv_point(d, E, pr, xv, yv, zv, ov);
+            qx[lane] = xv; qy[lane] = yv; qz[lane] = zv; qo[lane] = ov;
+        }
+        if (lane < BV_NB) {
+            double xv, yv, zv, ov;
+            bv_point(d, E, rt0 * BV_TILE + lane, xv, yv, zv, ov);
+            const int t = G * BV_TILE + lane;
+            qx[t] = xv; qy[t] = yv; qz[t] = zv; qo[t] = ov;
+        }
+        if (lane == 0) s_kp = E.kp;
+        bv_wave_sync();
+    }
+    bv_d4 acc[G][2];
+    const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
+    {
+        const bv_kparams &kp = s_kp;
+        const bool planar = (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 ||
+                             KIND == BV_KIND_NU25) && kp.metric == 0 && !kp.use_z;
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            acc[s][0] = zero;
+            acc[s][1] = zero;
+            if (s >= nr) continue;
+            const int rt = row0 + s;
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                if (rt == rt0 && ct == 1) continue; /* upper tile of the diagonal block */
+                if (KIND == BV_KIND_LOADA) {
+                    acc[s][ct] = -*(const bv_d4 *)(ws + bv_tile_off(rt, rt0 + ct, T) + 4 * lane);
+                } else {
+                    const int p = rt * BV_TILE + lr;
+                    const int ip = s * BV_TILE + lr;
+                    const int q0 = (rt0 + ct) * BV_TILE;
+                    bv_d4 v;
+                    if (planar && rt * BV_TILE + 15 < m && q0 + 15 < m) {
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
+                            v[g] = bv_matern_closed<KIND>(
+                                bv_dist_euclid(qx[ip], qy[ip], 0.0, qx[iq], qy[iq], 0.0, 0), kp);
+                        }
+                    } else {
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
+                            v[g] = bv_aug_entry<KIND>(p, q0 + 4 * g + lq, m, qx[ip], qy[ip],
+                                                      qz[ip], qx[iq], qy[iq], qz[iq], qo[iq], kp);
+                        }
+                    }
+                    acc[s][ct] = -v;
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if (j >= 1) {
+        /* k = 2j-2, 2j-1 */
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                for (int s = 0; s < G; ++s) {
+                    if (s < nr) {
+                        acc[s][0] = bv_mfma(jl[0][kk][g], xl[s][kk][g], acc[s][0]);
+                        acc[s][1] = bv_mfma(jl[1][kk][g], xl[s][kk][g], acc[s][1]);
+                    }
+                }
+            }
+        }
+    }
+    if (j >= 2) {
+        switch (nr) {
+        case 1: bv_step_gemm<1, G>(ws, T, j, rt0 - 2, row0, lane, acc); break;
+#if BV_STEP_ROWS >= 2
+        case 2: bv_step_gemm<2, G>(ws, T, j, rt0 - 2, row0, lane, acc); break;
+#endif
+#if BV_STEP_ROWS >= 3
+        case 3: bv_step_gemm<3, G>(ws, T, j, rt0 - 2, row0, lane, acc); break;
+#endif
+#if BV_STEP_ROWS >= 4
+        case 4: bv_step_gemm<4, G>(ws, T, j, rt0 - 2, row0, lane, acc); break;
+#endif
+        default: break;
+        }
+    }
+
+    /* (c) store C, or factor the diagonal block */
+    if (!lead) {
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            if (s < nr) {
+                const int rt = row0 + s;
+                *(bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane) = -acc[s][0];
+                *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane) = -acc[s][1];
+            }
+        }
+        return;
+    }
+    /* lead: rows rt0, rt0+1 are s = 0, 1 (nr >= 2: the row count 2P - 2j is
+       even); its further rows store C as the other items do */
+#pragma unroll
+    for (int s = 2; s < G; ++s) {
+        if (s < nr) {
+            const int rt = row0 + s;
+            *(bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane) = -acc[s][0];
+            *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane) = -acc[s][1];
+        }
+    }
+    *(bv_d4 *)(sW + 0 * 256 + 4 * lane) = -acc[0][0];
+    *(bv_d4 *)(sW + 1 * 256 + 4 * lane) = -acc[1][0];
+    *(bv_d4 *)(sW + 2 * 256 + 4 * lane) = -acc[1][1];
+    if (lane == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; }
+    bv_wave_sync();
+    bv_diag_factor_tiles(sW, sW + 256, sW + 512, lane, rt0 * BV_TILE, m, cs0, sCol, sRow, sT, sW,
+                         sPiv, &s_dotd, &s_info);
+    bv_panel_logdet(lane, rt0 * BV_TILE, m, cs0, sPiv, &s_logdet);
+    bv_wave_sync();
+    *(bv_d4 *)(ws + bv_tile_off(rt0, rt0, T) + 4 * lane) = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
+    *(bv_d4 *)(ws + bv_tile_off(rt0 + 1, rt0, T) + 4 * lane) =
+        *(const bv_d4 *)(sW + 1 * 256 + 4 * lane);
+    *(bv_d4 *)(ws + bv_tile_off(rt0 + 1, rt0 + 1, T) + 4 * lane) =
+        *(const bv_d4 *)(sW + 2 * 256 + 4 * lane);
+    if (lane == 0) {
+        const int64_t o = (int64_t)it.desc * pmax + j;
+        part_logdet[o] = s_logdet;
+        part_dot[o] = s_dotd; /* row m in the diagonal block; else the solve of
+                                 step j+1 overwrites it */
+        part_info[o] = s_info;
+    }
+}
+
+/* Per-block totals of the stepped factorization's partials, fixed order. */
+__global__ void __launch_bounds__(256)
+bv_step_parts_kernel(int ndesc, const bv_block_desc *__restrict__ desc,
+                     const bv_eval_slot *__restrict__ slots, const double *part_logdet,
+                     const double *part_dot, const int32_t *part_info, int pmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndesc) return;
+    const bv_block_desc d = desc[i];
+    const bv_eval_slot &E = slots[d.eval];
+    const int P = d.mp / BV_NB;
+    double ld = 0.0, dt = 0.0;
+    int info = 0;
+    for (int j = 0; j < P; ++j) {
+        const int64_t o = (int64_t)i * pmax + j;
+        ld += part_logdet[o];
+        dt += part_dot[o];
+        if (info == 0 && part_info[o] != 0) info = part_info[o];
+    }
+    E.logdet[d.block] = ld;
+    E.dot[d.block] = dt;
+    E.info[d.block] = info;
+}
+
 /* Writes the augmented matrix A of every block (lower tiles, accumulator
    layout) into the workspace, for the kinds whose entries are too costly to
    recompute inside the factorization (general nu, space-time).  grid.x: tile
@@ -734,6 +1061,40 @@ hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_
     else
         hipLaunchKernelGGL(bv_gen_kernel<BV_KIND_GENERAL>, dim3(max_tiles, nblk), dim3(256), 0,
                            st, desc, slots, ws);
+    return hipGetLastError();
+}
+
+hipError_t bv_launch_step(int kind, unsigned nitems, int j, hipStream_t st,
+                          const bv_step_item *items, const bv_block_desc *desc,
+                          const bv_eval_slot *slots, double *ws, double *part_logdet,
+                          double *part_dot, int32_t *part_info, int pmax) {
+    if (nitems == 0) return hipSuccess;
+    switch (kind) {
+    case BV_KIND_NU05:
+        hipLaunchKernelGGL(bv_step_kernel<BV_KIND_NU05>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    case BV_KIND_NU15:
+        hipLaunchKernelGGL(bv_step_kernel<BV_KIND_NU15>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    case BV_KIND_NU25:
+        hipLaunchKernelGGL(bv_step_kernel<BV_KIND_NU25>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    default:
+        hipLaunchKernelGGL(bv_step_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t bv_launch_step_parts(int ndesc, hipStream_t st, const bv_block_desc *desc,
+                                const bv_eval_slot *slots, const double *part_logdet,
+                                const double *part_dot, const int32_t *part_info, int pmax) {
+    hipLaunchKernelGGL(bv_step_parts_kernel, dim3((ndesc + 255) / 256), dim3(256), 0, st, ndesc,
+                       desc, slots, part_logdet, part_dot, part_info, pmax);
     return hipGetLastError();
 }
 
