@@ -145,6 +145,14 @@ def summarize_traffic(fetch_csv, write_csv, mib):
     print(json.dumps(d, indent=1))
 
 
+def eng_pmax(blocks, cs):
+    """Panel steps of the largest block: Mp / 32 with Mp = roundup(m + 1, 32),
+    m = cs + bs (block 0: bs alone)."""
+    bn = blocks.batch_num
+    m = max(int(bn[0]), int(bn[1:].max()) + cs if len(bn) > 1 else 0)
+    return ((m + 1 + 31) // 32 * 32) // 32
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +189,14 @@ def main():
     st, bad = eng.status()
     if st != bv.BV_OK:
         raise SystemExit("evaluation failed: status %d block %d" % (st, bad))
+    cfg = eng.factor_config()
+    if cfg is None or cfg[1] == 0:
+        kname = "bv_block_kernel (nu=1/2 closed form, %s wavefront%s per block)" % (
+            "one" if cfg and cfg[0] == 1 else "four", "" if cfg and cfg[0] == 1 else "s")
+    else:
+        kname = "bv_step_kernel x %d panel-step launches (nu=1/2 closed form, %d chunk(s))" % (
+            eng_pmax(blocks, cs), cfg[2])
+    log("factorization configuration (autotuned): %s" % (cfg,))
     try:
         import torch
         if torch.cuda.is_available():
@@ -270,7 +286,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
-            "kernel": "bv_block_kernel (nu=1/2 closed form)",
+            "kernel": kname,
+            "config": cfg,
             "flops_per_launch": F,
             "kernel_ms": kern_ms,
         },

@@ -81,7 +81,7 @@ EXPORTED = [
     "bv_last_status", "bv_last_kernel_ms", "bv_iterations", "bv_destroy", "bv_matern",
     "bv_partition", "bv_blocks_free", "bv_device_selftest",
     "bv_server_create", "bv_server_attach", "bv_server_submit", "bv_server_poll",
-    "bv_server_pending", "bv_server_destroy",
+    "bv_server_pending", "bv_server_destroy", "bv_factor_config",
 ]
 
 _lib = None
@@ -126,6 +126,8 @@ def load_library(path=None):
     lib.bv_blocks_free.restype = None
     lib.bv_device_selftest.argtypes = [ctypes.c_int]
     lib.bv_device_selftest.restype = ctypes.c_int
+    lib.bv_factor_config.argtypes = [ctypes.c_void_p, _ip]
+    lib.bv_factor_config.restype = ctypes.c_int
     lib.bv_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_void_p)]
     lib.bv_server_create.restype = ctypes.c_int
@@ -279,6 +281,13 @@ class Engine:
 
     def iterations(self):
         return self._lib.bv_iterations(self.ctx)
+
+    def factor_config(self):
+        """(variant, stepped, chunks, streams) of the factorization in use,
+        or None before the first evaluation has chosen it."""
+        cfg = np.zeros(4, np.int32)
+        rc = self._lib.bv_factor_config(self.ctx, cfg.ctypes.data_as(_ip))
+        return tuple(int(v) for v in cfg) if rc == BV_OK else None
 
     def close(self):
         if getattr(self, "ctx", None):

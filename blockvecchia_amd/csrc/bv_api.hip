@@ -87,6 +87,7 @@ struct bv_ctx {
     bv_step_item *d_items = nullptr;
     std::vector<int64_t> step_off;          /* [chunk][pmax + 1] item offsets      */
     int nchunks = 1, nstreams = 1;          /* block chunks, and streams they cycle over */
+    int autotune = 1, tuned = 0;            /* choose the configuration on first use */
     hipStream_t cstream[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t cev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int pmax = 0;
@@ -256,6 +257,18 @@ static int bv_make_kparams(const bv_ctx *c, const double *theta, bv_kparams *kp,
     return BV_OK;
 }
 
+/* The side streams (and their events) of the chunked stepped factorization,
+   created on first use. */
+static int bv_side_streams(bv_ctx *c) {
+    for (int i = 0; i < c->nstreams; ++i)
+        if (!c->cstream[i] && hipStreamCreateWithFlags(&c->cstream[i], hipStreamNonBlocking) != hipSuccess)
+            return BV_ERR_HIP;
+    for (int i = 0; i <= c->nstreams; ++i)
+        if (!c->cev[i] && hipEventCreateWithFlags(&c->cev[i], hipEventDisableTiming) != hipSuccess)
+            return BV_ERR_HIP;
+    return BV_OK;
+}
+
 extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     if (!p || !out) return BV_ERR_ARG;
     *out = nullptr;
@@ -287,6 +300,8 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     if (const char *v = getenv("BV_PATH")) c->stepped = (strcmp(v, "stepped") == 0) ? 1 : 0;
     if (const char *v = getenv("BV_CHUNKS")) c->nchunks = std::max(1, std::min(64, atoi(v)));
     if (const char *v = getenv("BV_CHUNK_STREAMS")) c->nstreams = std::max(1, std::min(4, atoi(v)));
+    if (getenv("BV_FACTOR_KERNEL") || getenv("BV_PATH") || getenv("BV_CHUNKS")) c->autotune = 0;
+    if (const char *v = getenv("BV_AUTOTUNE")) c->autotune = c->autotune && atoi(v) != 0;
     c->h_batch_num.assign(p->batch_num, p->batch_num + p->batch_count);
     c->h_accum.assign((size_t)c->B + 1, 0);
     for (int i = 0; i < c->B; ++i) {
@@ -326,17 +341,9 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
         bv_destroy(c);
         return BV_ERR_HIP;
     }
-    if (c->stepped && c->nstreams > 1) {
-        for (int i = 0; i < c->nstreams; ++i)
-            if (hipStreamCreateWithFlags(&c->cstream[i], hipStreamNonBlocking) != hipSuccess) {
-                bv_destroy(c);
-                return BV_ERR_HIP;
-            }
-        for (int i = 0; i <= c->nstreams; ++i)
-            if (hipEventCreateWithFlags(&c->cev[i], hipEventDisableTiming) != hipSuccess) {
-                bv_destroy(c);
-                return BV_ERR_HIP;
-            }
+    if (c->stepped && c->nstreams > 1 && bv_side_streams(c) != BV_OK) {
+        bv_destroy(c);
+        return BV_ERR_HIP;
     }
     int rc = BV_OK;
     const size_t n = (size_t)c->n;
@@ -371,8 +378,10 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     return BV_OK;
 }
 
-/* Launch k evaluations (thetas row-major, nparam each) and wait for them. */
-static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *out, int *status) {
+/* Launch k evaluations (thetas row-major, nparam each) with the current
+   factorization configuration and wait for them. */
+static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, double *out,
+                       int *status) {
     if (k <= 0 || !thetas || !out) return BV_ERR_ARG;
     if (nparam != bv_nparams(c->kernel)) return BV_ERR_ARG;
     BV_HIP(hipSetDevice(c->device));
@@ -476,6 +485,61 @@ static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *ou
     }
     c->last_status = worst;
     return BV_OK;
+}
+
+/* Factorization configurations the autotuner chooses from: the fused kernel
+   with four wavefronts or one wavefront per block, and the stepped
+   factorization, whole or in two chunks over two streams.  Which is fastest
+   depends on the block sizes and the covariance kind. */
+struct bv_cfg {
+    int variant, stepped, nchunks, nstreams;
+};
+static const bv_cfg bv_cfgs[] = {{0, 0, 1, 1}, {1, 0, 1, 1}, {0, 1, 1, 1}, {0, 1, 2, 2}};
+
+static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
+    const bool relayout = g.nchunks != c->nchunks;
+    c->variant = g.variant;
+    c->stepped = g.stepped;
+    c->nchunks = g.nchunks;
+    c->nstreams = g.nstreams;
+    if (relayout) c->desc_evals = 0; /* the step items depend on the chunks */
+    if (c->stepped && c->nstreams > 1) return bv_side_streams(c);
+    return BV_OK;
+}
+
+/* bv_run_once, choosing the configuration on the first call: each candidate
+   runs once to warm up and twice timed (device time of the factorization),
+   the fastest is kept for the life of the context, and the call's results
+   come from a final run with it. */
+static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *out, int *status) {
+    if (c->autotune && !c->tuned) {
+        int best = 0;
+        double best_ms = INFINITY;
+        const int ncfg = (int)(sizeof(bv_cfgs) / sizeof(bv_cfgs[0]));
+        for (int i = 0; i < ncfg; ++i) {
+            int rc = bv_apply_cfg(c, bv_cfgs[i]);
+            if (rc) return rc;
+            double ms = INFINITY;
+            for (int r = 0; r < 3; ++r) {
+                if ((rc = bv_run_once(c, k, nparam, thetas, out, status))) return rc;
+                if (r > 0) ms = std::min(ms, c->last_ms);
+            }
+            if (ms < best_ms) { best_ms = ms; best = i; }
+        }
+        int rc = bv_apply_cfg(c, bv_cfgs[best]);
+        if (rc) return rc;
+        c->tuned = 1;
+    }
+    return bv_run_once(c, k, nparam, thetas, out, status);
+}
+
+extern "C" int bv_factor_config(bv_ctx *c, int32_t *cfg) {
+    if (!c || !cfg) return BV_ERR_ARG;
+    cfg[0] = c->variant;
+    cfg[1] = c->stepped;
+    cfg[2] = c->nchunks;
+    cfg[3] = c->nstreams;
+    return c->tuned || !c->autotune ? BV_OK : BV_ERR_BUSY;
 }
 
 extern "C" double bv_llh(unsigned n, const double *theta, double *grad, void *f_data) {

@@ -108,6 +108,15 @@ int bv_last_status(bv_ctx *ctx, int *first_bad_block);
 /* Device time in milliseconds of the last evaluation's kernels (HIP events). */
 double bv_last_kernel_ms(bv_ctx *ctx);
 
+/* The factorization configuration in use: cfg[0] = kernel variant (0: four
+ * wavefronts per block, 1: one), cfg[1] = 1 for the stepped factorization
+ * (one launch per panel step over all blocks), cfg[2] = block chunks,
+ * cfg[3] = streams.  No counterpart in the reference.  The first evaluation
+ * of a context times the candidate configurations on its own data and keeps
+ * the fastest (unless BV_FACTOR_KERNEL, BV_PATH or BV_CHUNKS fix it, or
+ * BV_AUTOTUNE=0); before that, returns BV_ERR_BUSY. */
+int bv_factor_config(bv_ctx *ctx, int32_t *cfg);
+
 /* Number of iterations (objective calls) so far; the reference's
  * data->iterations counter. */
 int bv_iterations(bv_ctx *ctx);

@@ -98,8 +98,9 @@ def test_server_large_request_runs_alone_and_rejects_bad_theta():
     while srv.pending() > 0:
         for t, v, st in srv.poll(wait=True):
             got[t] = v
-    assert got[1] == engines[0].llh([1.5, 0.1, 0.5])
-    assert got[2] == engines[1].llh([1.5, 0.1, 0.5])
+    for tag, e in ((1, engines[0]), (2, engines[1])):
+        alone = e.llh([1.5, 0.1, 0.5])
+        assert abs(got[tag] - alone) <= LLH_RTOL * abs(alone), (tag, got[tag], alone)
     srv.close()
     for e in engines:
         e.close()
