@@ -15,17 +15,14 @@ same chain, timed on this host's cores).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
 
-HBM traffic of the factorization kernel (the `traffic` field) comes from two
-rocprofv3 counter passes over this script, each with a plain device copy of
-known size appended as the calibration (--calib-copy-mib), summarised by
---traffic-from into profiles/bv_block_kernel_n20000_cs300_traffic.json:
-
-    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- \
-        python bench.py --steps 5 --warmup 1 --no-cpu-baseline --calib-copy-mib 512
-    rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- \
-        python bench.py --steps 5 --warmup 1 --no-cpu-baseline --calib-copy-mib 512
-    python bench.py --traffic-from OUT/fetch/run_counter_collection.csv \
-        OUT/write/run_counter_collection.csv --calib-copy-mib 512
+HBM traffic of the factorization launches (the `traffic` field, per
+evaluation) comes from two rocprofv3 counter passes over this script with the
+factorization configuration fixed by environment (the autotuner's choice),
+each with a plain device copy of known size appended as the calibration
+(--calib-copy-mib), summarised by --traffic-from into
+profiles/factorization_n20000_cs300_traffic.json; the commands are in the
+profile summary next to it.  It is reported only when the configuration in
+use matches the one it was measured with.
 """
 import argparse
 import json
@@ -65,19 +62,22 @@ def make_workload(n=20000, bc=1500, cs=300, theta=(1.5, 0.1, 0.5), seed=0):
     return blocks
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the factorization kernel from the committed
-    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
-    calibration), or None."""
-    p = os.path.join(ROOT, "profiles", "bv_block_kernel_n20000_cs300_traffic.json")
+def traffic_from_profiles(config):
+    """HBM bytes per evaluation of the factorization launches from the
+    committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a
+    copy-kernel calibration), or None when it was measured for another
+    factorization configuration than the one in use."""
+    p = os.path.join(ROOT, "profiles", "factorization_n20000_cs300_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
         d = json.load(f)
+    if config is None or list(d.get("config", [])) != list(config):
+        return None
     return d.get("hbm_bytes_per_launch")
 
 
-TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bv_block_kernel_n20000_cs300_traffic.json")
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "factorization_n20000_cs300_traffic.json")
 
 
 def calibration_copy(mib, repeats=3):
@@ -95,47 +95,48 @@ def calibration_copy(mib, repeats=3):
     torch.cuda.synchronize()
 
 
-def summarize_traffic(fetch_csv, write_csv, mib):
-    """Per-dispatch FETCH_SIZE and WRITE_SIZE (KiB) of bv_block_kernel and of
-    the calibration copy, from rocprofv3 counter CSVs; the kernel's HBM bytes
-    are the counters scaled by copy bytes / copy counter."""
+FACTOR_PREFIXES = ("void bv_block_kernel<", "void bv_step_kernel<", "bv_step_parts_kernel",
+                   "void bv_gen_kernel<")
+
+
+def summarize_traffic(fetch_csv, write_csv, mib, config):
+    """HBM bytes per evaluation of the factorization launches (bv_block_kernel,
+    or the bv_step_kernel panel steps with their partial sums, plus the
+    covariance generation launch when one runs) from rocprofv3 FETCH_SIZE /
+    WRITE_SIZE CSVs, per evaluation = summed over those dispatches / the
+    number of bv_reduce_kernel dispatches; scaled by copy bytes / the copy's
+    counter."""
     import csv
 
-    def per_dispatch(path, counter):
-        vals = {}
+    def per_eval(path, counter):
+        fac, copy, nred = {}, {}, set()
         for r in csv.DictReader(open(path)):
             if r["Counter_Name"] != counter:
                 continue
-            name = r["Kernel_Name"]
-            if name.startswith("void bv_block_kernel<"):
-                key = "kernel"
+            name, did, v = r["Kernel_Name"], int(r["Dispatch_Id"]), float(r["Counter_Value"])
+            if name.startswith(FACTOR_PREFIXES):
+                fac[did] = fac.get(did, 0.0) + v
+            elif name.startswith("bv_reduce_kernel"):
+                nred.add(did)
             elif "copy" in name.lower() or "elementwise" in name.lower():
-                key = "copy"
-            else:
-                continue
-            did = int(r["Dispatch_Id"])
-            vals.setdefault(key, {}).setdefault(did, 0.0)
-            vals[key][did] += float(r["Counter_Value"])
-        out = {}
-        for key, d in vals.items():
-            ids = sorted(d)
-            if key == "copy":
-                ids = ids[-3:]  # the calibration copies run last
-            out[key] = sum(d[i] for i in ids) / len(ids)
-        return out
+                copy[did] = copy.get(did, 0.0) + v
+        ids = sorted(copy)[-3:]  # the calibration copies run last
+        return sum(fac.values()) / len(nred), sum(copy[i] for i in ids) / len(ids), len(nred)
 
-    f = per_dispatch(fetch_csv, "FETCH_SIZE")
-    w = per_dispatch(write_csv, "WRITE_SIZE")
+    f, fc, ne = per_eval(fetch_csv, "FETCH_SIZE")
+    w, wc, _ = per_eval(write_csv, "WRITE_SIZE")
     copy_bytes = mib * (1 << 20)
-    f_scale = copy_bytes / (f["copy"] * 1024.0)
-    w_scale = copy_bytes / (w["copy"] * 1024.0)
-    fetch = f["kernel"] * 1024.0 * f_scale
-    write = w["kernel"] * 1024.0 * w_scale
+    f_scale = copy_bytes / (fc * 1024.0)
+    w_scale = copy_bytes / (wc * 1024.0)
+    fetch = f * 1024.0 * f_scale
+    write = w * 1024.0 * w_scale
     d = {
-        "kernel": "bv_block_kernel<0> (nu=1/2), n=20000 bc=1500 cs=300, 1478 blocks",
-        "fetch_kib_per_dispatch": f["kernel"], "write_kib_per_dispatch": w["kernel"],
+        "workload": "n=20000 bc=1500 cs=300 (1478 blocks), nu=1/2",
+        "config": list(config),
+        "evaluations": ne,
+        "fetch_kib_per_eval": f, "write_kib_per_eval": w,
         "copy_bytes": copy_bytes,
-        "copy_fetch_kib": f["copy"], "copy_write_kib": w["copy"],
+        "copy_fetch_kib": fc, "copy_write_kib": wc,
         "fetch_scale": f_scale, "write_scale": w_scale,
         "hbm_fetch_bytes_per_launch": fetch, "hbm_write_bytes_per_launch": write,
         "hbm_bytes_per_launch": fetch + write,
@@ -166,9 +167,14 @@ def main():
                     help="append a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 counter CSVs into the traffic JSON and exit")
+    ap.add_argument("--config", default=None,
+                    help="with --traffic-from: the factorization configuration the counters "
+                         "were taken with, as variant,stepped,chunks,streams")
     args = ap.parse_args()
     if args.traffic_from:
-        summarize_traffic(args.traffic_from[0], args.traffic_from[1], args.calib_copy_mib or 512)
+        cfg = [int(v) for v in (args.config or "0,0,1,1").split(",")]
+        summarize_traffic(args.traffic_from[0], args.traffic_from[1], args.calib_copy_mib or 512,
+                          cfg)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the likelihood shards by blocks "
@@ -255,7 +261,7 @@ def main():
                  r["latency_ms"]["p50"], r["latency_ms"]["p99"]))
         log("serving scenarios: %.1f s" % (time.perf_counter() - t0))
 
-    traffic = traffic_from_profiles()
+    traffic = traffic_from_profiles(cfg)
     out = {
         "metric": METRIC,
         "value": evals_per_s,
