@@ -28,9 +28,10 @@
  * split into 4 products, the g-th using column 4g + l/16 of every lane, which
  * is where both operands hold it.
  *
- * bv_reduce_kernel: llh = sum_i -1/2 (dot_i + logdet_i + bs_i log(2 pi)) in
- * block order (include/llh_Xvecchia_batch.h:480-494), plus the first block
- * whose factorization failed.
+ * bv_reduce_kernel: llh = sum_i -1/2 (dot_i + logdet_i + bs_i log(2 pi))
+ * (include/llh_Xvecchia_batch.h:480-494 sums it in block order; here in a
+ * fixed pairwise order, equal to rounding), plus the first block whose
+ * factorization failed.
  */
 #include <hip/hip_runtime.h>
 #include "bv_engine.h"
@@ -951,26 +952,28 @@ bv_reduce_kernel(const bv_eval_slot *__restrict__ slots) {
     __shared__ int s_bad[BV_THREADS];
     const int tid = threadIdx.x;
     const int B = E.nblocks;
-    const int per = (B + BV_THREADS - 1) / BV_THREADS;
-    const int lo = tid * per, hi = (lo + per < B) ? lo + per : B;
+    /* thread t: blocks t, t + 256, ... (coalesced), then a fixed pairwise
+       tree: the order of the sum depends only on B */
     double acc = 0.0;
-    int bad = -1;
-    for (int k = lo; k < hi; ++k) {
+    int bad = INT32_MAX;
+    for (int k = tid; k < B; k += BV_THREADS) {
         acc += -(E.dot[k] + E.logdet[k] + (double)E.batch_num[k] * E.log2pi) * 0.5;
-        if (bad < 0 && E.info[k] != 0) bad = k;
+        if (bad == INT32_MAX && E.info[k] != 0) bad = k;
     }
     s_part[tid] = acc;
     s_bad[tid] = bad;
     __syncthreads();
-    if (tid == 0) {
-        double llh = 0.0;
-        int first = -1;
-        for (int t = 0; t < BV_THREADS; ++t) {
-            llh += s_part[t];
-            if (first < 0 && s_bad[t] >= 0) first = s_bad[t];
+#pragma unroll
+    for (int off = BV_THREADS / 2; off >= 1; off >>= 1) {
+        if (tid < off) {
+            s_part[tid] += s_part[tid + off];
+            s_bad[tid] = s_bad[tid] < s_bad[tid + off] ? s_bad[tid] : s_bad[tid + off];
         }
-        E.result[0] = llh;
-        E.result[1] = (double)first;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        E.result[0] = s_part[0];
+        E.result[1] = s_bad[0] == INT32_MAX ? -1.0 : (double)s_bad[0];
     }
 }
 
