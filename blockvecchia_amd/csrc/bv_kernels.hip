@@ -929,18 +929,26 @@ bv_gen_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__rest
     const int T = d.mp / BV_TILE;
     const int ntile = T * (T + 1) / 2;
     int tix = blockIdx.x;
-    if (tix >= ntile) return;
+    if (tix >= ntile) return; /* whole workgroup */
+    const bv_eval_slot &E = slots[d.eval];
+    constexpr bool TAB = (KIND == BV_KIND_GENERAL || KIND == BV_KIND_SPACETIME);
+    __shared__ bv_knu_tab s_tab;
+    if (TAB) {
+        bv_knu_tab_fill(E.kp.nu, &s_tab, (int)threadIdx.x, 256);
+        __syncthreads();
+    }
     int ct = 0;
     while (tix >= T - ct) { tix -= T - ct; ++ct; }
     const int rt = ct + tix;
-    const bv_eval_slot &E = slots[d.eval];
     const int e = threadIdx.x, lane = e >> 2, g = e & 3;
     const int p = rt * BV_TILE + (lane & 15);
     const int q = ct * BV_TILE + 4 * g + (lane >> 4);
     double xp, yp, zp, op, xq, yq, zq, oq;
     bv_point(d, E, p, xp, yp, zp, op);
     bv_point(d, E, q, xq, yq, zq, oq);
-    const double v = bv_aug_entry<KIND>(p, q, d.m, xp, yp, zp, xq, yq, zq, oq, E.kp);
+    double v;
+    if (TAB && p < d.m && q < d.m) v = bv_cov_tab<KIND>(xp, yp, zp, xq, yq, zq, E.kp, s_tab);
+    else v = bv_aug_entry<KIND>(p, q, d.m, xp, yp, zp, xq, yq, zq, oq, E.kp);
     ws_base[d.ws_off + bv_tile_off(rt, ct, T) + e] = v;
 }
 
@@ -982,6 +990,11 @@ extern "C" __global__ void __launch_bounds__(256)
 bv_matern_kernel(int m, int n, const double *x1, const double *y1, const double *z1,
                  const double *x2, const double *y2, const double *z2, bv_kparams kp, int kind,
                  double *A) {
+    __shared__ bv_knu_tab s_tab;
+    if (kind == BV_KIND_GENERAL || kind == BV_KIND_SPACETIME) {
+        bv_knu_tab_fill(kp.nu, &s_tab, (int)threadIdx.x, (int)blockDim.x);
+        __syncthreads();
+    }
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)m * n) return;
     const int i = (int)(e % m), jj = (int)(e / m);
@@ -992,9 +1005,11 @@ bv_matern_kernel(int m, int n, const double *x1, const double *y1, const double 
     case BV_KIND_NU15: v = bv_cov<BV_KIND_NU15>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp); break;
     case BV_KIND_NU25: v = bv_cov<BV_KIND_NU25>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp); break;
     case BV_KIND_SPACETIME:
-        v = bv_cov<BV_KIND_SPACETIME>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp);
+        v = bv_cov_tab<BV_KIND_SPACETIME>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp, s_tab);
         break;
-    default: v = bv_cov<BV_KIND_GENERAL>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp); break;
+    default:
+        v = bv_cov_tab<BV_KIND_GENERAL>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp, s_tab);
+        break;
     }
     A[e] = v;
 }

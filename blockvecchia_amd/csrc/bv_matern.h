@@ -146,6 +146,128 @@ BV_HD double bv_besselk(double nu, double x) {
     return k0;
 }
 
+/* ---- K_nu with the nu-only work done once --------------------------------
+ * Everything that depends only on nu (mu, N, the Temme coefficients
+ * 1/Gamma(1 +- mu), pi mu / sin(pi mu), the reciprocals of the series'
+ * denominators k^2 - mu^2, k - mu, k + mu, k, and the backward-recurrence
+ * coefficients) is tabulated once per covariance parameter set, so the
+ * per-entry work is one log, two exp, a few divisions and multiply-adds.
+ * Same algorithm and accuracy as bv_besselk. */
+#define BV_KT 24 /* Temme terms: for x <= 2 the series converges in < 16 */
+#define BV_KC 74 /* backward recurrence: n <= 12 + 120 / x < 73 for x > 2 */
+struct bv_knu_tab {
+    double mu, g1, g2, inv_rgp, inv_rgm, pim, nu;
+    int32_t nl, pad_;
+    double rk2[BV_KT + 1], rkm[BV_KT + 1], rkp[BV_KT + 1], rk[BV_KT + 1];
+    double an[BV_KC + 1], bn[BV_KC + 1]; /* (n+1/2)^2 - mu^2, ((n-1/2)^2 - mu^2) / n */
+};
+
+/* Fill the entries i0, i0 + stride, ... of the table (threads of a
+   workgroup share the work); i0 == 0 also fills the scalars. */
+BV_HD void bv_knu_tab_fill(double nu, bv_knu_tab *t, int i0, int stride) {
+    const int nl = (int)(nu + 0.5);
+    const double mu = nu - (double)nl, mu2 = mu * mu;
+    if (i0 == 0) {
+        const double a[BV_RGAMMA_NCOEF] = BV_RGAMMA_COEFS;
+        double g1 = 0.0, g2 = 0.0;
+        for (int k = BV_RGAMMA_NCOEF - 1; k >= 1; k -= 2) g1 = g1 * mu2 + a[k];
+        for (int k = BV_RGAMMA_NCOEF - 2; k >= 0; k -= 2) g2 = g2 * mu2 + a[k];
+        g1 = -g1;
+        t->mu = mu;
+        t->nu = nu;
+        t->nl = nl;
+        t->g1 = g1;
+        t->g2 = g2;
+        t->inv_rgp = 1.0 / (g2 - mu * g1);
+        t->inv_rgm = 1.0 / (g2 + mu * g1);
+        t->pim = (fabs(mu) < 1e-15) ? 1.0 : (BV_PI * mu) / sin(BV_PI * mu);
+    }
+    for (int k = i0; k <= BV_KT; k += stride) {
+        const double dk = (double)k;
+        t->rk2[k] = k ? 1.0 / (dk * dk - mu2) : 0.0;
+        t->rkm[k] = k ? 1.0 / (dk - mu) : 0.0;
+        t->rkp[k] = k ? 1.0 / (dk + mu) : 0.0;
+        t->rk[k] = k ? 1.0 / dk : 0.0;
+    }
+    for (int n = i0; n <= BV_KC; n += stride) {
+        const double dn = (double)n;
+        t->an[n] = (dn + 0.5) * (dn + 0.5) - mu2;
+        t->bn[n] = n ? ((dn - 0.5) * (dn - 0.5) - mu2) / dn : 0.0;
+    }
+}
+
+/* x^nu K_nu(x) for x > 0 from the table; lx_out = log x. */
+BV_HD double bv_xnu_knu_tab(double x, const bv_knu_tab &t) {
+    const double mu = t.mu;
+    const double lnx = log(x);
+    const double inv_x = 1.0 / x;
+    double k0, k1;
+    if (x <= 2.0) {
+        /* Temme's series (bv_knu_temme) */
+        const double lx = 0.6931471805599453 - lnx; /* log(2 / x) */
+        const double sig = mu * lx;
+        const double es = exp(sig), ies = 1.0 / es;
+        double shs; /* sinh(sig) / sig */
+        if (fabs(sig) < 0.5) {
+            /* sum_k sig^2k / (2k+1)!, 9 terms (the rest < 1e-19 relative) */
+            const double s2 = sig * sig;
+            double h = 1.0 / 6402373705728000.0; /* 1 / 19! */
+            h = h * s2 + 1.0 / 355687428096000.0;  /* 1 / 17! */
+            h = h * s2 + 1.0 / 1307674368000.0;    /* 1 / 15! */
+            h = h * s2 + 1.0 / 6227020800.0;       /* 1 / 13! */
+            h = h * s2 + 1.0 / 39916800.0;         /* 1 / 11! */
+            h = h * s2 + 1.0 / 362880.0;           /* 1 / 9! */
+            h = h * s2 + 1.0 / 5040.0;             /* 1 / 7! */
+            h = h * s2 + 1.0 / 120.0;              /* 1 / 5! */
+            h = h * s2 + 1.0 / 6.0;                /* 1 / 3! */
+            shs = h * s2 + 1.0;
+        } else {
+            shs = 0.5 * (es - ies) / sig;
+        }
+        double f = t.pim * (0.5 * (es + ies) * t.g1 + shs * lx * t.g2);
+        double p = 0.5 * es * t.inv_rgp;
+        double q = 0.5 * ies * t.inv_rgm;
+        double c = 1.0;
+        double sk = f, sk1 = p;
+        const double x2 = 0.25 * x * x;
+        for (int k = 1; k <= BV_KT; ++k) {
+            const double dk = (double)k;
+            f = (dk * f + p + q) * t.rk2[k];
+            p = p * t.rkm[k];
+            q = q * t.rkp[k];
+            c = c * x2 * t.rk[k];
+            const double tt = c * f;
+            sk += tt;
+            sk1 += c * (p - dk * f);
+            if (fabs(tt) < 1e-17 * fabs(sk)) break;
+        }
+        k0 = sk;
+        k1 = sk1 * 2.0 * inv_x;
+    } else {
+        /* backward-recurrence CF2 (bv_knu_cf2) */
+        const int nmax = 12 + (int)(120.0 * inv_x);
+        double un1 = 0.0, un = 1.0, tt = 1.0;
+        for (int n = nmax; n >= 1; --n) {
+            const double dn = (double)n;
+            const double um1 = 2.0 * (dn + x) * un - t.an[n] * un1;
+            tt = um1 + t.bn[n] * tt;
+            un1 = un;
+            un = um1;
+            if (fabs(un) > 1e200) { un1 *= 1e-200; un *= 1e-200; tt *= 1e-200; }
+        }
+        const double sinv = un / tt;
+        const double k = sqrt(BV_PI * 0.5 * inv_x) * exp(-x) * sinv;
+        k0 = k;
+        k1 = k * inv_x * (mu + 0.5 + x + (mu * mu - 0.25) * un1 / un);
+    }
+    for (int k = 1; k <= t.nl; ++k) {
+        const double kn = k0 + 2.0 * (mu + (double)k) * inv_x * k1;
+        k0 = k1;
+        k1 = kn;
+    }
+    return exp(t.nu * lnx) * k0;
+}
+
 /* Great-circle distance (src/ckernel.cpp:145-154). */
 BV_HD double bv_dist_earth(double lat1d, double lon1d, double lat2d, double lon2d) {
     const double lat1r = lat1d * BV_PI / 180.0, lon1r = lon1d * BV_PI / 180.0;
@@ -208,6 +330,31 @@ BV_HD double bv_cov(double x1, double y1, double z1, double x2, double y2, doubl
     e = e / kp.dist_scale;
     if (e == 0.0) return kp.sigma2;
     return kp.con * pow(e, kp.nu) * bv_besselk(kp.nu, e);
+}
+
+/* bv_cov for the kinds that need K_nu (GENERAL, SPACETIME), with the
+   nu-only work taken from a table filled by bv_knu_tab_fill(kp.nu, ...). */
+template <int KIND>
+BV_HD double bv_cov_tab(double x1, double y1, double z1, double x2, double y2, double z2,
+                        const bv_kparams &kp, const bv_knu_tab &t) {
+    if (KIND == BV_KIND_SPACETIME) {
+        const double d = bv_dist_euclid(x1, y1, z1, x2, y2, z2, 1);
+        const double e = d / kp.beta;
+        const double dt = fabs(z1 - z2);
+        const double base = pow(dt, 2.0 * kp.nu_t) / kp.beta_t + 1.0;
+        const double e2 = pow(base, kp.sep / 2.0);
+        const double e3 = e / e2;
+        const double e4 = pow(base, kp.sep + kp.aux);
+        if (e == 0.0) return kp.sigma2 / e4;
+        return kp.con * bv_xnu_knu_tab(e3, t) / e4;
+    }
+    double d;
+    if (kp.metric == 1) d = bv_dist_earth(x1, y1, x2, y2);
+    else d = bv_dist_euclid(x1, y1, z1, x2, y2, z2, kp.use_z);
+    double e = d / kp.beta;
+    e = e / kp.dist_scale;
+    if (e == 0.0) return kp.sigma2;
+    return kp.con * bv_xnu_knu_tab(e, t);
 }
 
 #endif /* BV_MATERN_H */

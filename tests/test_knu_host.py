@@ -17,6 +17,11 @@ ROOT = os.path.dirname(HERE)
 SHIM = r'''
 #include "bv_matern.h"
 extern "C" double shim_besselk(double nu, double x) { return bv_besselk(nu, x); }
+extern "C" double shim_xnu_knu_tab(double nu, double x) {
+    static bv_knu_tab t;
+    bv_knu_tab_fill(nu, &t, 0, 1);
+    return bv_xnu_knu_tab(x, t);
+}
 extern "C" double shim_cov(int kind, double x1, double y1, double x2, double y2, double s2,
                            double beta, double nu, double con) {
     bv_kparams kp = {};
@@ -44,6 +49,8 @@ def shim(tmp_path_factory):
     lib = ctypes.CDLL(str(so))
     lib.shim_besselk.argtypes = [ctypes.c_double, ctypes.c_double]
     lib.shim_besselk.restype = ctypes.c_double
+    lib.shim_xnu_knu_tab.argtypes = [ctypes.c_double, ctypes.c_double]
+    lib.shim_xnu_knu_tab.restype = ctypes.c_double
     lib.shim_cov.argtypes = [ctypes.c_int] + [ctypes.c_double] * 8
     lib.shim_cov.restype = ctypes.c_double
     return lib
@@ -83,3 +90,29 @@ def test_engine_closed_forms_vs_general(shim):
             a = shim.shim_cov(kind, *p, 1.3, 0.07, nu, con)
             b = shim.shim_cov(3, *p, 1.3, 0.07, nu, con)
             assert abs(a - b) <= 1e-14 * abs(b)
+
+
+def test_tabulated_knu_golden_and_random(shim):
+    """x^nu K_nu(x) from the nu-tables (the path the covariance generation
+    uses for general nu) against the mpmath table and the oracle."""
+    import math
+    with open(os.path.join(HERE, "golden", "besselk_golden.json")) as f:
+        rows = json.load(f)["rows"]
+    worst = 0.0
+    for nu, x, ref in rows:
+        ref = float(ref)
+        if ref == 0.0 or nu == 0.0:
+            continue
+        v = shim.shim_xnu_knu_tab(nu, x)
+        r = math.exp(nu * math.log(x)) * ref
+        if r == 0.0 or not math.isfinite(r):
+            continue
+        worst = max(worst, abs(v - r) / r)
+    assert worst < 2e-14, worst
+    rng = np.random.default_rng(21)
+    for _ in range(400):
+        nu = rng.uniform(0.01, 3.0)
+        x = 10 ** rng.uniform(-7, np.log10(650))
+        a = shim.shim_xnu_knu_tab(nu, x)
+        b = math.exp(nu * math.log(x)) * oracle.besselk(nu, x)
+        assert abs(a - b) <= 2e-14 * b, (nu, x, a, b)
