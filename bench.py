@@ -72,7 +72,9 @@ def traffic_from_profiles(config):
         return None
     with open(p) as f:
         d = json.load(f)
-    if config is None or list(d.get("config", [])) != list(config):
+    # the kernels (and so their traffic) depend on the first four entries
+    # only; the fifth says whether they are replayed from a graph
+    if config is None or list(d.get("config", []))[:4] != list(config)[:4]:
         return None
     return d.get("hbm_bytes_per_launch")
 
@@ -200,8 +202,8 @@ def main():
         kname = "bv_block_kernel (nu=1/2 closed form, %s wavefront%s per block)" % (
             "one" if cfg and cfg[0] == 1 else "four", "" if cfg and cfg[0] == 1 else "s")
     else:
-        kname = "bv_step_kernel x %d panel-step launches (nu=1/2 closed form, %d chunk(s))" % (
-            eng_pmax(blocks, cs), cfg[2])
+        kname = "bv_step_kernel x %d panel-step launches (nu=1/2 closed form, %d chunk(s)%s)" % (
+            eng_pmax(blocks, cs), cfg[2], ", HIP graph replay" if cfg[4] else "")
     log("factorization configuration (autotuned): %s" % (cfg,))
     try:
         import torch

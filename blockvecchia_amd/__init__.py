@@ -81,7 +81,7 @@ EXPORTED = [
     "bv_last_status", "bv_last_kernel_ms", "bv_iterations", "bv_destroy", "bv_matern",
     "bv_partition", "bv_blocks_free", "bv_device_selftest",
     "bv_server_create", "bv_server_attach", "bv_server_submit", "bv_server_poll",
-    "bv_server_pending", "bv_server_destroy", "bv_factor_config",
+    "bv_server_pending", "bv_server_destroy", "bv_factor_config", "bv_set_graph",
 ]
 
 _lib = None
@@ -128,6 +128,8 @@ def load_library(path=None):
     lib.bv_device_selftest.restype = ctypes.c_int
     lib.bv_factor_config.argtypes = [ctypes.c_void_p, _ip]
     lib.bv_factor_config.restype = ctypes.c_int
+    lib.bv_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.bv_set_graph.restype = ctypes.c_int
     lib.bv_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_void_p)]
     lib.bv_server_create.restype = ctypes.c_int
@@ -283,11 +285,18 @@ class Engine:
         return self._lib.bv_iterations(self.ctx)
 
     def factor_config(self):
-        """(variant, stepped, chunks, streams) of the factorization in use,
-        or None before the first evaluation has chosen it."""
-        cfg = np.zeros(4, np.int32)
+        """(variant, stepped, chunks, streams, graph) of the factorization in
+        use, or None before the first evaluation has chosen it."""
+        cfg = np.zeros(5, np.int32)
         rc = self._lib.bv_factor_config(self.ctx, cfg.ctypes.data_as(_ip))
         return tuple(int(v) for v in cfg) if rc == BV_OK else None
+
+    def set_graph(self, enable):
+        """Replay single evaluations from a captured HIP graph (default) or
+        launch their kernels one by one."""
+        rc = self._lib.bv_set_graph(self.ctx, 1 if enable else 0)
+        if rc != BV_OK:
+            raise RuntimeError("bv_set_graph failed: %d" % rc)
 
     def close(self):
         if getattr(self, "ctx", None):

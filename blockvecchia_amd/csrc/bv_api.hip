@@ -93,6 +93,11 @@ struct bv_ctx {
     int pmax = 0;
     double *d_part_logdet = nullptr, *d_part_dot = nullptr;
     int32_t *d_part_info = nullptr;
+    /* single evaluations replay a captured graph of their launches when
+       allowed (use_graph) and, once tuned, when that timed faster */
+    int use_graph = 1, graph_wins = 1;
+    hipGraphExec_t graph = nullptr;
+    int64_t graph_key[10] = {};
 };
 
 static int bv_nparams(int kernel) {
@@ -302,6 +307,7 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     if (const char *v = getenv("BV_CHUNK_STREAMS")) c->nstreams = std::max(1, std::min(4, atoi(v)));
     if (getenv("BV_FACTOR_KERNEL") || getenv("BV_PATH") || getenv("BV_CHUNKS")) c->autotune = 0;
     if (const char *v = getenv("BV_AUTOTUNE")) c->autotune = c->autotune && atoi(v) != 0;
+    if (const char *v = getenv("BV_GRAPH")) c->use_graph = atoi(v) != 0;
     c->h_batch_num.assign(p->batch_num, p->batch_num + p->batch_count);
     c->h_accum.assign((size_t)c->B + 1, 0);
     for (int i = 0; i < c->B; ++i) {
@@ -378,45 +384,12 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     return BV_OK;
 }
 
-/* Launch k evaluations (thetas row-major, nparam each) with the current
-   factorization configuration and wait for them. */
-static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, double *out,
-                       int *status) {
-    if (k <= 0 || !thetas || !out) return BV_ERR_ARG;
-    if (nparam != bv_nparams(c->kernel)) return BV_ERR_ARG;
-    BV_HIP(hipSetDevice(c->device));
-    int rc;
-    if ((rc = bv_reserve(c, k)) || (rc = bv_layout(c, k))) return rc;
-    const size_t B = (size_t)c->B;
-    for (int e = 0; e < k; ++e) {
-        bv_eval_slot &s = c->h_slots[e];
-        memset(&s, 0, sizeof(s));
-        if ((rc = bv_make_kparams(c, thetas + (size_t)e * nparam, &s.kp, &s.kind))) return rc;
-        s.x = c->d_x; s.y = c->d_y; s.z = c->d_z; s.obs = c->d_obs;
-        s.nbr = c->d_nbr; s.cx = c->d_cx; s.cy = c->d_cy; s.cz = c->d_cz; s.cobs = c->d_cobs;
-        s.cs = c->cs;
-        s.nblocks = c->B;
-        s.batch_num = c->d_batch_num;
-        s.logdet = c->d_logdet + e * B;
-        s.dot = c->d_dot + e * B;
-        s.info = c->d_info + e * B;
-        s.result = c->d_result + 4 * e;
-        s.log2pi = log(2 * 3.141592653589793);
-    }
+/* The kernels of one call after its slots are on the device: covariance
+   generation where it is tabulated beforehand, the factorization in the
+   configuration chosen, and the stepped form's partial sums.  Everything is
+   ordered on c->stream (side streams fork from it and join back). */
+static int bv_enqueue(bv_ctx *c, unsigned nblk, int kinds_used) {
     const size_t lds = sizeof(double) * 4 * (size_t)c->max_mp;
-    BV_HIP(hipEventRecord(c->ev0, c->stream));
-    /* one kind per launch: evaluations of a call share the kernel when their
-       smoothness matches; otherwise one launch per kind over its own slots */
-    int kinds_used = 0;
-    for (int e = 0; e < k; ++e) kinds_used |= 1 << c->h_slots[e].kind;
-    if (kinds_used & (kinds_used - 1)) {
-        /* smoothness differs across the call: all go through the general path */
-        for (int e = 0; e < k; ++e) c->h_slots[e].kind = BV_KIND_GENERAL;
-        kinds_used = 1 << BV_KIND_GENERAL;
-    }
-    BV_HIP(hipMemcpyAsync(c->d_slots, c->h_slots, sizeof(bv_eval_slot) * k,
-                          hipMemcpyHostToDevice, c->stream));
-    const unsigned nblk = (unsigned)(B * k);
     for (int kind = 0; kind < 8; ++kind) {
         if (!(kinds_used & (1 << kind))) continue;
         if (kind == BV_KIND_GENERAL || kind == BV_KIND_SPACETIME) {
@@ -458,6 +431,97 @@ static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, doubl
         else
             BV_HIP(bv_launch_block(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
     }
+    return BV_OK;
+}
+
+/* Identity of the launches bv_enqueue makes: the configuration, the kinds and
+   every device pointer and count baked into them. */
+static void bv_graph_key(const bv_ctx *c, int kinds_used, int64_t *key) {
+    key[0] = kinds_used;
+    key[1] = c->variant | (c->stepped << 4) | (c->nchunks << 8) | ((int64_t)c->nstreams << 16);
+    key[2] = (int64_t)(intptr_t)c->d_ws;
+    key[3] = (int64_t)(intptr_t)c->d_desc;
+    key[4] = (int64_t)(intptr_t)c->d_items;
+    key[5] = (int64_t)(intptr_t)c->d_slots;
+    key[6] = (int64_t)(intptr_t)c->d_part_logdet;
+    key[7] = c->desc_evals;
+    key[8] = c->pmax;
+    key[9] = c->max_mp;
+}
+
+/* bv_enqueue through a graph captured on first use and replayed while its
+   key holds.  The graph holds kernel launches only (no memory set or copy
+   nodes); the per-call parameters reach it through the slots, copied
+   before the launch. */
+static int bv_enqueue_graph(bv_ctx *c, unsigned nblk, int kinds_used) {
+    int64_t key[10];
+    bv_graph_key(c, kinds_used, key);
+    if (c->graph && memcmp(key, c->graph_key, sizeof(key)) != 0) {
+        hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    if (!c->graph) {
+        BV_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = bv_enqueue(c, nblk, kinds_used);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        BV_HIP(ec);
+        const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        BV_HIP(ei);
+        memcpy(c->graph_key, key, sizeof(key));
+    }
+    BV_HIP(hipGraphLaunch(c->graph, c->stream));
+    return BV_OK;
+}
+
+/* Launch k evaluations (thetas row-major, nparam each) with the current
+   factorization configuration and wait for them. */
+static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, double *out,
+                       int *status) {
+    if (k <= 0 || !thetas || !out) return BV_ERR_ARG;
+    if (nparam != bv_nparams(c->kernel)) return BV_ERR_ARG;
+    BV_HIP(hipSetDevice(c->device));
+    int rc;
+    if ((rc = bv_reserve(c, k)) || (rc = bv_layout(c, k))) return rc;
+    const size_t B = (size_t)c->B;
+    for (int e = 0; e < k; ++e) {
+        bv_eval_slot &s = c->h_slots[e];
+        memset(&s, 0, sizeof(s));
+        if ((rc = bv_make_kparams(c, thetas + (size_t)e * nparam, &s.kp, &s.kind))) return rc;
+        s.x = c->d_x; s.y = c->d_y; s.z = c->d_z; s.obs = c->d_obs;
+        s.nbr = c->d_nbr; s.cx = c->d_cx; s.cy = c->d_cy; s.cz = c->d_cz; s.cobs = c->d_cobs;
+        s.cs = c->cs;
+        s.nblocks = c->B;
+        s.batch_num = c->d_batch_num;
+        s.logdet = c->d_logdet + e * B;
+        s.dot = c->d_dot + e * B;
+        s.info = c->d_info + e * B;
+        s.result = c->d_result + 4 * e;
+        s.log2pi = log(2 * 3.141592653589793);
+    }
+    BV_HIP(hipEventRecord(c->ev0, c->stream));
+    /* one kind per launch: evaluations of a call share the kernel when their
+       smoothness matches; otherwise one launch per kind over its own slots */
+    int kinds_used = 0;
+    for (int e = 0; e < k; ++e) kinds_used |= 1 << c->h_slots[e].kind;
+    if (kinds_used & (kinds_used - 1)) {
+        /* smoothness differs across the call: all go through the general path */
+        for (int e = 0; e < k; ++e) c->h_slots[e].kind = BV_KIND_GENERAL;
+        kinds_used = 1 << BV_KIND_GENERAL;
+    }
+    BV_HIP(hipMemcpyAsync(c->d_slots, c->h_slots, sizeof(bv_eval_slot) * k,
+                          hipMemcpyHostToDevice, c->stream));
+    const unsigned nblk = (unsigned)(B * k);
+    if (c->use_graph && c->graph_wins && k == 1 && (c->tuned || !c->autotune))
+        rc = bv_enqueue_graph(c, nblk, kinds_used);
+    else
+        rc = bv_enqueue(c, nblk, kinds_used);
+    if (rc) return rc;
     BV_HIP(hipEventRecord(c->ev1, c->stream));
     BV_HIP(bv_launch_reduce((unsigned)k, c->stream, c->d_slots));
     BV_HIP(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * 4 * k,
@@ -529,6 +593,19 @@ static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *ou
         int rc = bv_apply_cfg(c, bv_cfgs[best]);
         if (rc) return rc;
         c->tuned = 1;
+        if (k == 1 && c->use_graph) {
+            /* a graph launch costs more than one kernel launch and less than
+               many: replay only if it times faster than launching directly */
+            double t[2] = {INFINITY, INFINITY};
+            for (int g = 0; g < 2; ++g) {
+                c->graph_wins = g;
+                for (int r = 0; r < 3; ++r) {
+                    if ((rc = bv_run_once(c, k, nparam, thetas, out, status))) return rc;
+                    if (r > 0) t[g] = std::min(t[g], c->last_ms);
+                }
+            }
+            c->graph_wins = t[1] < t[0];
+        }
     }
     return bv_run_once(c, k, nparam, thetas, out, status);
 }
@@ -539,7 +616,14 @@ extern "C" int bv_factor_config(bv_ctx *c, int32_t *cfg) {
     cfg[1] = c->stepped;
     cfg[2] = c->nchunks;
     cfg[3] = c->nstreams;
+    cfg[4] = c->use_graph && c->graph_wins;
     return c->tuned || !c->autotune ? BV_OK : BV_ERR_BUSY;
+}
+
+extern "C" int bv_set_graph(bv_ctx *c, int enable) {
+    if (!c) return BV_ERR_ARG;
+    c->use_graph = enable != 0;
+    return BV_OK;
 }
 
 extern "C" double bv_llh(unsigned n, const double *theta, double *grad, void *f_data) {
@@ -602,6 +686,7 @@ extern "C" void bv_destroy(bv_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph) hipGraphExecDestroy(c->graph);
     for (int i = 0; i < 4; ++i)
         if (c->cstream[i]) { hipStreamSynchronize(c->cstream[i]); hipStreamDestroy(c->cstream[i]); }
     for (int i = 0; i < 5; ++i)

@@ -921,8 +921,11 @@ bv_step_parts_kernel(int ndesc, const bv_block_desc *__restrict__ desc,
    layout) into the workspace, for the kinds whose entries are too costly to
    recompute inside the factorization (general nu, space-time).  grid.x: tile
    index within the block, grid.y: descriptor. */
+#ifndef BV_GEN_WAVES_PER_SIMD
+#define BV_GEN_WAVES_PER_SIMD 1
+#endif
 template <int KIND>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, BV_GEN_WAVES_PER_SIMD)
 bv_gen_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
               double *ws_base) {
     const bv_block_desc d = desc[blockIdx.y];

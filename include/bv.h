@@ -111,11 +111,23 @@ double bv_last_kernel_ms(bv_ctx *ctx);
 /* The factorization configuration in use: cfg[0] = kernel variant (0: four
  * wavefronts per block, 1: one), cfg[1] = 1 for the stepped factorization
  * (one launch per panel step over all blocks), cfg[2] = block chunks,
- * cfg[3] = streams.  No counterpart in the reference.  The first evaluation
- * of a context times the candidate configurations on its own data and keeps
- * the fastest (unless BV_FACTOR_KERNEL, BV_PATH or BV_CHUNKS fix it, or
- * BV_AUTOTUNE=0); before that, returns BV_ERR_BUSY. */
+ * cfg[3] = streams, cfg[4] = 1 when single evaluations replay a HIP graph
+ * (cfg holds 5 entries).  No counterpart in the reference.  The first
+ * evaluation of a context times the candidate configurations on its own data
+ * and keeps the fastest, then times graph replay against direct launches for
+ * single evaluations (unless BV_FACTOR_KERNEL, BV_PATH or BV_CHUNKS fix the
+ * configuration, or BV_AUTOTUNE=0); before that, returns BV_ERR_BUSY. */
 int bv_factor_config(bv_ctx *ctx, int32_t *cfg);
+
+/* Single evaluations (bv_llh, bv_llh_many with k = 1) may replay a HIP graph
+ * of their kernel launches, captured on the first such call after the
+ * factorization configuration is fixed and again whenever the configuration
+ * or the buffers change; the autotuner keeps it only where it times faster
+ * (many launches: the stepped factorization over several streams).
+ * enable = 0 launches them one by one (as calls with k > 1 always do); the
+ * results are the same.  Default on; BV_GRAPH=0 in the environment at
+ * bv_create turns it off.  No counterpart in the reference. */
+int bv_set_graph(bv_ctx *ctx, int enable);
 
 /* Number of iterations (objective calls) so far; the reference's
  * data->iterations counter. */
