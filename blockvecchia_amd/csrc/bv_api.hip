@@ -808,6 +808,10 @@ struct bv_server {
     int device = 0;
     double ws_budget = 0.0;                  /* bytes of workspace per lane  */
     int max_evals = 64;                      /* evaluations per batch        */
+    /* kind groups up to this padded size use the one-wavefront factorization
+       kernel: merged batches hold many blocks, and one wavefront per block
+       keeps more of them resident (measured faster at every size tried) */
+    int64_t one_wave_max_mp = INT32_MAX;
     std::vector<bv_ctx *> datasets;
     std::vector<bv_request> queue;           /* FIFO (front = oldest)        */
     size_t qhead = 0;
@@ -866,6 +870,7 @@ extern "C" int bv_server_create(int device, int nlanes, double ws_bytes_per_lane
     s->device = device;
     s->ws_budget = ws_bytes_per_lane;
     s->max_evals = max_evals_per_batch;
+    if (const char *v = getenv("BV_SERVER_1W_MAXMP")) s->one_wave_max_mp = atoi(v);
     s->lanes.resize(nlanes);
     for (bv_lane &L : s->lanes) {
         if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
@@ -982,7 +987,10 @@ static int bv_server_launch(bv_server *s, bv_lane &L) {
             const unsigned T = (unsigned)(kmax_mp[kd] / BV_TILE);
             BV_HIP(bv_launch_gen(kd, T * (T + 1) / 2, nblk, L.stream, dd, L.d_slots, L.d_ws));
         }
-        BV_HIP(bv_launch_block(kd, nblk, lds, L.stream, dd, L.d_slots, L.d_ws));
+        if (kmax_mp[kd] <= s->one_wave_max_mp)
+            BV_HIP(bv_launch_block_1w(kd, nblk, lds, L.stream, dd, L.d_slots, L.d_ws));
+        else
+            BV_HIP(bv_launch_block(kd, nblk, lds, L.stream, dd, L.d_slots, L.d_ws));
     }
     BV_HIP(bv_launch_reduce((unsigned)k, L.stream, L.d_slots));
     BV_HIP(hipMemcpyAsync(L.h_result, L.d_result, sizeof(double) * 4 * k, hipMemcpyDeviceToHost,

@@ -142,8 +142,10 @@ void bv_destroy(bv_ctx *ctx);
  * HBM); requests queue in arrival order; bv_server_poll admits them into idle
  * lanes (one stream each) while their workspace bytes fit the lane budget,
  * merges their blocks into shared launches, and returns finished results.
- * Calls on one server are not thread-safe; a result does not depend on the
- * batch it ran in. */
+ * Merged launches factor with one wavefront per block (BV_SERVER_1W_MAXMP=M
+ * in the environment at creation limits that to kind groups of padded size
+ * <= M; larger ones use four wavefronts).  Calls on one server are not
+ * thread-safe; a result does not depend on the batch it ran in. */
 typedef struct bv_server bv_server;
 
 /* nlanes concurrent batches (1..16), each with ws_bytes_per_lane of
