@@ -238,6 +238,8 @@ extern "C" int bv_partition(int64_t n, const double *x, const double *y, const d
         cen = pts;
         clusterNum.assign((size_t)n, 1);
     } else {
+        /* k-means assigns every point to a cluster only from its first epoch on */
+        if (epochs < 1) return BV_ERR_ARG;
         cen = init_centroids(pts, (int)nclusters, seed);
         kmeans(pts, cen, epochs, (int)nclusters, threads);
         int maxid = 0;

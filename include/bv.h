@@ -205,8 +205,9 @@ typedef struct bv_blocks {
  * when n < 2 bc), random centroid order (srand(42) Fisher-Yates), reorderIndex,
  * firstbatch, cluster_combine, cluster_to_batch and findNearestPoints.  The
  * k-means is a deterministic restatement (zeroed accumulators, z = 0 in 2-D,
- * sums in point order).  kmeans_epochs: 50 in the reference.  threads: host
- * threads (<= 0: all).  Returns BV_OK and fills *out. */
+ * sums in point order).  kmeans_epochs: 50 in the reference; at least 1
+ * when k-means runs (n >= 2 bc), else BV_ERR_ARG.  threads: host threads
+ * (<= 0: all).  Returns BV_OK and fills *out. */
 int bv_partition(int64_t n, const double *x, const double *y, const double *z,
                  const double *obs, int32_t bc, int32_t cs, int32_t seed,
                  int32_t kmeans_epochs, int32_t time_flag, int32_t distance_metric,

@@ -66,3 +66,14 @@ def test_classic_vecchia_one_point_blocks():
     b = bv.partition(x, y, np.zeros(100), 80, 10, seed=2)
     assert b.batch_num[0] >= 10
     assert np.all(b.batch_num[1:] == 1)
+
+
+def test_zero_kmeans_epochs_rejected():
+    """k-means with no epoch assigns no point to a cluster: an argument
+    error, not a partition (the classic-Vecchia path needs no epoch)."""
+    x, y = datagen.uniform_locations(400, 0)
+    obs = np.zeros(400)
+    with pytest.raises(RuntimeError, match="bv_partition failed"):
+        bv.partition(x, y, obs, 20, 30, seed=0, epochs=0)
+    b = bv.partition(x, y, obs, 300, 30, seed=0, epochs=0)  # n < 2 bc: one point per cluster
+    assert int(np.sum(b.batch_num)) == 400
