@@ -82,6 +82,7 @@ EXPORTED = [
     "bv_partition", "bv_blocks_free", "bv_device_selftest",
     "bv_server_create", "bv_server_attach", "bv_server_submit", "bv_server_poll",
     "bv_server_pending", "bv_server_destroy", "bv_factor_config", "bv_set_graph",
+    "bv_partition_gpu",
 ]
 
 _lib = None
@@ -122,6 +123,8 @@ def load_library(path=None):
                                  ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                  ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(BvBlocks)]
     lib.bv_partition.restype = ctypes.c_int
+    lib.bv_partition_gpu.argtypes = [ctypes.c_int32] + list(lib.bv_partition.argtypes)
+    lib.bv_partition_gpu.restype = ctypes.c_int
     lib.bv_blocks_free.argtypes = [ctypes.POINTER(BvBlocks)]
     lib.bv_blocks_free.restype = None
     lib.bv_device_selftest.argtypes = [ctypes.c_int]
@@ -183,14 +186,21 @@ class Blocks:
 
 
 def partition(x, y, obs, bc, cs, seed=0, z=None, epochs=50, time_flag=False, metric=0,
-              threads=0):
+              threads=0, device=None):
     """Host pre-processing (k-means / classic Vecchia, random block order,
-    firstbatch, kNN conditioning sets) in libbv.  Returns Blocks."""
+    firstbatch, kNN conditioning sets) in libbv.  device=None: all on the
+    host (bv_partition); device=d: the k-means assignment steps on GPU d
+    (bv_partition_gpu), with the same result.  Returns Blocks."""
     lib = load_library()
     x, y, obs, z = _f64(x), _f64(y), _f64(obs), _f64(z)
     res = BvBlocks()
-    rc = lib.bv_partition(len(x), _d(x), _d(y), _d(z), _d(obs), bc, cs, seed, epochs,
-                          1 if time_flag else 0, metric, threads, ctypes.byref(res))
+    if device is None:
+        rc = lib.bv_partition(len(x), _d(x), _d(y), _d(z), _d(obs), bc, cs, seed, epochs,
+                              1 if time_flag else 0, metric, threads, ctypes.byref(res))
+    else:
+        rc = lib.bv_partition_gpu(int(device), len(x), _d(x), _d(y), _d(z), _d(obs), bc, cs,
+                                  seed, epochs, 1 if time_flag else 0, metric, threads,
+                                  ctypes.byref(res))
     if rc != BV_OK:
         raise RuntimeError("bv_partition failed: %d" % rc)
     try:

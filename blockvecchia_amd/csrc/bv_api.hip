@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "bv.h"
+#include "bv_prep.h"
 #include "bv_engine.h"
 
 hipError_t bv_launch_block(int kind, unsigned nblk, size_t lds, hipStream_t st,
@@ -38,6 +39,9 @@ hipError_t bv_launch_matern(int m, int n, const double *x1, const double *y1, co
                             const double *x2, const double *y2, const double *z2,
                             const bv_kparams &kp, int kind, double *A);
 hipError_t bv_launch_selftest(const double *a, const double *b, double *d);
+hipError_t bv_launch_kmeans_assign(int64_t n, const double *px, const double *py, const double *pz,
+                                   int k, const double *cx, const double *cy, const double *cz,
+                                   int32_t *cluster, hipStream_t st);
 
 #define BV_HIP(call)                                                                    \
     do {                                                                                \
@@ -113,19 +117,19 @@ static int bv_upload(T **dst, const T *src, size_t count, hipStream_t s) {
 }
 
 static void bv_free_eval_buffers(bv_ctx *c) {
-    if (c->d_desc) hipFree(c->d_desc);
-    if (c->d_slots) hipFree(c->d_slots);
-    if (c->h_slots) hipHostFree(c->h_slots);
-    if (c->d_ws) hipFree(c->d_ws);
-    if (c->d_logdet) hipFree(c->d_logdet);
-    if (c->d_dot) hipFree(c->d_dot);
-    if (c->d_info) hipFree(c->d_info);
-    if (c->d_result) hipFree(c->d_result);
-    if (c->h_result) hipHostFree(c->h_result);
-    if (c->d_items) hipFree(c->d_items);
-    if (c->d_part_logdet) hipFree(c->d_part_logdet);
-    if (c->d_part_dot) hipFree(c->d_part_dot);
-    if (c->d_part_info) hipFree(c->d_part_info);
+    if (c->d_desc) (void)hipFree(c->d_desc);
+    if (c->d_slots) (void)hipFree(c->d_slots);
+    if (c->h_slots) (void)hipHostFree(c->h_slots);
+    if (c->d_ws) (void)hipFree(c->d_ws);
+    if (c->d_logdet) (void)hipFree(c->d_logdet);
+    if (c->d_dot) (void)hipFree(c->d_dot);
+    if (c->d_info) (void)hipFree(c->d_info);
+    if (c->d_result) (void)hipFree(c->d_result);
+    if (c->h_result) (void)hipHostFree(c->h_result);
+    if (c->d_items) (void)hipFree(c->d_items);
+    if (c->d_part_logdet) (void)hipFree(c->d_part_logdet);
+    if (c->d_part_dot) (void)hipFree(c->d_part_dot);
+    if (c->d_part_info) (void)hipFree(c->d_part_info);
     c->d_items = nullptr; c->d_part_logdet = nullptr; c->d_part_dot = nullptr;
     c->d_part_info = nullptr;
     c->d_desc = nullptr; c->d_slots = nullptr; c->h_slots = nullptr; c->d_ws = nullptr;
@@ -216,7 +220,7 @@ static int bv_layout(bv_ctx *c, int k) {
         }
         c->step_off[(size_t)ch * (c->pmax + 1) + c->pmax] = (int64_t)items.size();
     }
-    if (c->d_items) hipFree(c->d_items);
+    if (c->d_items) (void)hipFree(c->d_items);
     c->d_items = nullptr;
     if (!items.empty()) {
         BV_HIP(hipMalloc((void **)&c->d_items, sizeof(bv_step_item) * items.size()));
@@ -457,7 +461,7 @@ static int bv_enqueue_graph(bv_ctx *c, unsigned nblk, int kinds_used) {
     int64_t key[10];
     bv_graph_key(c, kinds_used, key);
     if (c->graph && memcmp(key, c->graph_key, sizeof(key)) != 0) {
-        hipGraphExecDestroy(c->graph);
+        (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
     }
     if (!c->graph) {
@@ -466,12 +470,12 @@ static int bv_enqueue_graph(bv_ctx *c, unsigned nblk, int kinds_used) {
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(c->stream, &g);
         if (rc) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return rc;
         }
         BV_HIP(ec);
         const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         BV_HIP(ei);
         memcpy(c->graph_key, key, sizeof(key));
     }
@@ -528,7 +532,7 @@ static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, doubl
                           hipMemcpyDeviceToHost, c->stream));
     BV_HIP(hipStreamSynchronize(c->stream));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
     c->last_ms = ms;
     c->last_k = k;
     int worst = BV_OK;
@@ -684,22 +688,22 @@ extern "C" int bv_iterations(bv_ctx *c) { return c ? c->iterations : -1; }
 
 extern "C" void bv_destroy(bv_ctx *c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->graph) hipGraphExecDestroy(c->graph);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (int i = 0; i < 4; ++i)
-        if (c->cstream[i]) { hipStreamSynchronize(c->cstream[i]); hipStreamDestroy(c->cstream[i]); }
+        if (c->cstream[i]) { (void)hipStreamSynchronize(c->cstream[i]); (void)hipStreamDestroy(c->cstream[i]); }
     for (int i = 0; i < 5; ++i)
-        if (c->cev[i]) hipEventDestroy(c->cev[i]);
+        if (c->cev[i]) (void)hipEventDestroy(c->cev[i]);
     bv_free_eval_buffers(c);
     double *bufs[] = {c->d_x, c->d_y, c->d_z, c->d_obs, c->d_cx, c->d_cy, c->d_cz, c->d_cobs};
     for (double *b : bufs)
-        if (b) hipFree(b);
-    if (c->d_nbr) hipFree(c->d_nbr);
-    if (c->d_batch_num) hipFree(c->d_batch_num);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (b) (void)hipFree(b);
+    if (c->d_nbr) (void)hipFree(c->d_nbr);
+    if (c->d_batch_num) (void)hipFree(c->d_batch_num);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -738,7 +742,7 @@ extern "C" int bv_matern(int m, int nn, const double *x1, const double *y1, cons
     BV_HIP(bv_launch_matern(m, nn, d[0], d[1], d[2], d[3], d[4], d[5], kp, kind, d[6]));
     BV_HIP(hipMemcpy(A, d[6], sm * nn, hipMemcpyDeviceToHost));
     for (double *p : d)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     return BV_OK;
 }
 
@@ -755,7 +759,7 @@ extern "C" int bv_device_selftest(int device) {
     BV_HIP(hipMemcpy(d + 64, hb, sizeof(hb), hipMemcpyHostToDevice));
     BV_HIP(bv_launch_selftest(d, d + 64, d + 128));
     BV_HIP(hipMemcpy(hd, d + 128, sizeof(hd), hipMemcpyDeviceToHost));
-    hipFree(d);
+    (void)hipFree(d);
     /* expected D[i][j] = sum_k A[i][k] B[k][j], lane l holds D[4g + l/16][l%16] */
     for (int l = 0; l < 64; ++l)
         for (int g = 0; g < 4; ++g) {
@@ -819,16 +823,16 @@ struct bv_server {
 };
 
 static void bv_lane_free(bv_lane &L) {
-    if (L.d_ws) hipFree(L.d_ws);
-    if (L.d_desc) hipFree(L.d_desc);
-    if (L.h_desc) hipHostFree(L.h_desc);
-    if (L.d_slots) hipFree(L.d_slots);
-    if (L.h_slots) hipHostFree(L.h_slots);
-    if (L.d_logdet) hipFree(L.d_logdet);
-    if (L.d_dot) hipFree(L.d_dot);
-    if (L.d_info) hipFree(L.d_info);
-    if (L.d_result) hipFree(L.d_result);
-    if (L.h_result) hipHostFree(L.h_result);
+    if (L.d_ws) (void)hipFree(L.d_ws);
+    if (L.d_desc) (void)hipFree(L.d_desc);
+    if (L.h_desc) (void)hipHostFree(L.h_desc);
+    if (L.d_slots) (void)hipFree(L.d_slots);
+    if (L.h_slots) (void)hipHostFree(L.h_slots);
+    if (L.d_logdet) (void)hipFree(L.d_logdet);
+    if (L.d_dot) (void)hipFree(L.d_dot);
+    if (L.d_info) (void)hipFree(L.d_info);
+    if (L.d_result) (void)hipFree(L.d_result);
+    if (L.h_result) (void)hipHostFree(L.h_result);
     L.d_ws = nullptr; L.d_desc = nullptr; L.h_desc = nullptr; L.d_slots = nullptr;
     L.h_slots = nullptr; L.d_logdet = nullptr; L.d_dot = nullptr; L.d_info = nullptr;
     L.d_result = nullptr; L.h_result = nullptr;
@@ -1048,12 +1052,90 @@ extern "C" int bv_server_pending(bv_server *s) {
 
 extern "C" void bv_server_destroy(bv_server *s) {
     if (!s) return;
-    hipSetDevice(s->device);
+    (void)hipSetDevice(s->device);
     for (bv_lane &L : s->lanes) {
-        if (L.stream) hipStreamSynchronize(L.stream);
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
         bv_lane_free(L);
-        if (L.done) hipEventDestroy(L.done);
-        if (L.stream) hipStreamDestroy(L.stream);
+        if (L.done) (void)hipEventDestroy(L.done);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
     }
     delete s;
+}
+
+/* ---- pre-processing with the k-means assignment step on the device ------ */
+
+struct bv_km_dev {
+    hipStream_t st = nullptr;
+    int64_t n = 0;
+    int kcap = 0;
+    double *px = nullptr, *py = nullptr, *pz = nullptr;
+    double *cx = nullptr, *cy = nullptr, *cz = nullptr;
+    int32_t *cl = nullptr;
+};
+
+static int bv_km_assign(void *user, int k, const double *cx, const double *cy, const double *cz,
+                        int32_t *cluster) {
+    bv_km_dev &D = *(bv_km_dev *)user;
+    if (k < 1 || k > D.kcap) return BV_ERR_ARG;
+    BV_HIP(hipMemcpyAsync(D.cx, cx, sizeof(double) * k, hipMemcpyHostToDevice, D.st));
+    BV_HIP(hipMemcpyAsync(D.cy, cy, sizeof(double) * k, hipMemcpyHostToDevice, D.st));
+    BV_HIP(hipMemcpyAsync(D.cz, cz, sizeof(double) * k, hipMemcpyHostToDevice, D.st));
+    BV_HIP(bv_launch_kmeans_assign(D.n, D.px, D.py, D.pz, k, D.cx, D.cy, D.cz, D.cl, D.st));
+    BV_HIP(hipMemcpyAsync(cluster, D.cl, sizeof(int32_t) * D.n, hipMemcpyDeviceToHost, D.st));
+    BV_HIP(hipStreamSynchronize(D.st));
+    return BV_OK;
+}
+
+static void bv_km_free(bv_km_dev &D) {
+    double *b[] = {D.px, D.py, D.pz, D.cx, D.cy, D.cz};
+    for (double *p : b)
+        if (p) (void)hipFree(p);
+    if (D.cl) (void)hipFree(D.cl);
+    if (D.st) (void)hipStreamDestroy(D.st);
+    D = bv_km_dev();
+}
+
+static int bv_km_setup(bv_km_dev &D, int64_t n, const double *x, const double *y,
+                       const double *z, int32_t time_flag, int kcap) {
+    D.n = n;
+    D.kcap = kcap;
+    BV_HIP(hipStreamCreateWithFlags(&D.st, hipStreamNonBlocking));
+    const size_t nb = sizeof(double) * (size_t)n, kb = sizeof(double) * (size_t)kcap;
+    BV_HIP(hipMalloc((void **)&D.px, nb));
+    BV_HIP(hipMalloc((void **)&D.py, nb));
+    BV_HIP(hipMalloc((void **)&D.pz, nb));
+    BV_HIP(hipMalloc((void **)&D.cx, kb));
+    BV_HIP(hipMalloc((void **)&D.cy, kb));
+    BV_HIP(hipMalloc((void **)&D.cz, kb));
+    BV_HIP(hipMalloc((void **)&D.cl, sizeof(int32_t) * (size_t)n));
+    BV_HIP(hipMemcpyAsync(D.px, x, nb, hipMemcpyHostToDevice, D.st));
+    BV_HIP(hipMemcpyAsync(D.py, y, nb, hipMemcpyHostToDevice, D.st));
+    if (time_flag) {
+        BV_HIP(hipMemcpyAsync(D.pz, z, nb, hipMemcpyHostToDevice, D.st));
+    } else {
+        BV_HIP(hipMemsetAsync(D.pz, 0, nb, D.st)); /* z = 0 in 2-D, as on the host */
+    }
+    BV_HIP(hipStreamSynchronize(D.st));
+    return BV_OK;
+}
+
+extern "C" int bv_partition_gpu(int32_t device, int64_t n, const double *x, const double *y,
+                                const double *z, const double *obs, int32_t bc, int32_t cs,
+                                int32_t seed, int32_t epochs, int32_t time_flag, int32_t metric,
+                                int32_t threads, bv_blocks *out) {
+    if (!out || n <= 0 || n > 2147483647LL || !x || !y || !obs || bc <= 0) return BV_ERR_ARG;
+    if (time_flag && !z) return BV_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev)
+        return BV_ERR_NO_DEVICE;
+    BV_HIP(hipSetDevice(device));
+    bv_km_dev D;
+    const bool km = n >= 2 * (int64_t)bc; /* else classic Vecchia: no k-means */
+    int rc = BV_OK;
+    if (km) rc = bv_km_setup(D, n, x, y, z, time_flag, bc);
+    if (rc == BV_OK)
+        rc = bv_partition_run(n, x, y, z, obs, bc, cs, seed, epochs, time_flag, metric, threads,
+                              km ? bv_km_assign : nullptr, &D, out);
+    bv_km_free(D);
+    return rc;
 }

@@ -955,6 +955,59 @@ bv_gen_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__rest
     ws_base[d.ws_off + bv_tile_off(rt, ct, T) + e] = v;
 }
 
+/* The k-means assignment step of the host pre-processing (bv_prep.cpp) on
+   the device: one thread per point, the centroids staged through LDS in
+   tiles, the host's rounding (d = ((0 + dx*dx) + dy*dy) + dz*dz with
+   dx = point - centroid, no contraction) and its tie rule (the first minimum
+   wins), so the assignments are the host loop's. */
+#define BV_KM_TILE 1024
+extern "C" __global__ void __launch_bounds__(256)
+bv_kmeans_assign_kernel(int64_t n, const double *__restrict__ px, const double *__restrict__ py,
+                        const double *__restrict__ pz, int k, const double *__restrict__ cx,
+                        const double *__restrict__ cy, const double *__restrict__ cz,
+                        int32_t *__restrict__ cluster) {
+#pragma clang fp contract(off)
+    __shared__ double sx[BV_KM_TILE], sy[BV_KM_TILE], sz[BV_KM_TILE];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool live = i < n;
+    const double ax = live ? px[i] : 0.0, ay = live ? py[i] : 0.0, az = live ? pz[i] : 0.0;
+    double best = 0.0;
+    int bi = 0;
+    for (int base = 0; base < k; base += BV_KM_TILE) {
+        const int cnt = min(BV_KM_TILE, k - base);
+        __syncthreads();
+        for (int t = threadIdx.x; t < cnt; t += 256) {
+            sx[t] = cx[base + t];
+            sy[t] = cy[base + t];
+            sz[t] = cz[base + t];
+        }
+        __syncthreads();
+        int c0 = 0;
+        if (base == 0) {
+            const double dx = ax - sx[0], dy = ay - sy[0], dz = az - sz[0];
+            double d = 0.0;
+            d += dx * dx;
+            d += dy * dy;
+            d += dz * dz;
+            best = d;
+            c0 = 1;
+        }
+#pragma unroll 8
+        for (int c = c0; c < cnt; ++c) {
+            const double dx = ax - sx[c], dy = ay - sy[c], dz = az - sz[c];
+            double d = 0.0;
+            d += dx * dx;
+            d += dy * dy;
+            d += dz * dz;
+            if (d < best) {
+                best = d;
+                bi = base + c;
+            }
+        }
+    }
+    if (live) cluster[i] = bi;
+}
+
 /* One workgroup per evaluation: llh and first failing block, fixed order. */
 extern "C" __global__ void __launch_bounds__(BV_THREADS)
 bv_reduce_kernel(const bv_eval_slot *__restrict__ slots) {
@@ -1135,5 +1188,14 @@ hipError_t bv_launch_matern(int m, int n, const double *x1, const double *y1, co
 
 hipError_t bv_launch_selftest(const double *a, const double *b, double *d) {
     hipLaunchKernelGGL(bv_selftest_kernel, dim3(1), dim3(64), 0, 0, a, b, d);
+    return hipGetLastError();
+}
+
+hipError_t bv_launch_kmeans_assign(int64_t n, const double *px, const double *py, const double *pz,
+                                   int k, const double *cx, const double *cy, const double *cz,
+                                   int32_t *cluster, hipStream_t st) {
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(bv_kmeans_assign_kernel, dim3(grid), dim3(256), 0, st, n, px, py, pz, k,
+                       cx, cy, cz, cluster);
     return hipGetLastError();
 }

@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "bv.h"
+#include "bv_prep.h"
 
 namespace {
 
@@ -95,13 +96,28 @@ std::vector<Pt> init_centroids(const std::vector<Pt> &pts, int k, int seed) {
     return cen;
 }
 
-/* kmean_par with a fixed accumulation order */
-void kmeans(std::vector<Pt> &pts, std::vector<Pt> &cen, int epochs, int k, int threads) {
+/* kmean_par with a fixed accumulation order; the assignment step on the host
+   or through `assign` */
+int kmeans(std::vector<Pt> &pts, std::vector<Pt> &cen, int epochs, int k, int threads,
+           bv_assign_fn assign, void *user) {
     const int64_t n = (int64_t)pts.size();
     std::vector<double> sum((size_t)k * 3);
     std::vector<int> card(k);
+    std::vector<double> ccx, ccy, ccz;
+    std::vector<int32_t> cl;
+    if (assign) {
+        ccx.resize(k); ccy.resize(k); ccz.resize(k);
+        cl.resize((size_t)n);
+    }
     for (int ep = 0; ep < epochs; ++ep) {
-        parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
+        if (assign) {
+            for (int c = 0; c < k; ++c) {
+                ccx[c] = cen[c].c[0]; ccy[c] = cen[c].c[1]; ccz[c] = cen[c].c[2];
+            }
+            const int rc = assign(user, k, ccx.data(), ccy.data(), ccz.data(), cl.data());
+            if (rc != BV_OK) return rc;
+            for (int64_t i = 0; i < n; ++i) pts[i].cluster = cl[i];
+        } else parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
             for (int64_t i = lo; i < hi; ++i) {
                 Pt &p = pts[i];
                 double best = sqdist3(p.c, cen[0].c);
@@ -134,6 +150,7 @@ void kmeans(std::vector<Pt> &pts, std::vector<Pt> &cen, int epochs, int k, int t
             cen[c].cluster = -1;
         }
     }
+    return BV_OK;
 }
 
 inline double deg2rad(double d) { return d * 3.141592653589793 / 180; }
@@ -217,6 +234,14 @@ extern "C" int bv_partition(int64_t n, const double *x, const double *y, const d
                             const double *obs, int32_t bc, int32_t cs, int32_t seed,
                             int32_t epochs, int32_t time_flag, int32_t metric, int32_t threads,
                             bv_blocks *out) {
+    return bv_partition_run(n, x, y, z, obs, bc, cs, seed, epochs, time_flag, metric, threads,
+                            nullptr, nullptr, out);
+}
+
+int bv_partition_run(int64_t n, const double *x, const double *y, const double *z,
+                     const double *obs, int32_t bc, int32_t cs, int32_t seed, int32_t epochs,
+                     int32_t time_flag, int32_t metric, int32_t threads, bv_assign_fn assign,
+                     void *user, bv_blocks *out) {
     if (!out || n <= 0 || !x || !y || !obs || bc <= 0 || cs < 0 || epochs < 0) return BV_ERR_ARG;
     if (time_flag && !z) return BV_ERR_ARG;
     if (n > 2147483647LL) return BV_ERR_ARG;
@@ -241,7 +266,8 @@ extern "C" int bv_partition(int64_t n, const double *x, const double *y, const d
         /* k-means assigns every point to a cluster only from its first epoch on */
         if (epochs < 1) return BV_ERR_ARG;
         cen = init_centroids(pts, (int)nclusters, seed);
-        kmeans(pts, cen, epochs, (int)nclusters, threads);
+        const int krc = kmeans(pts, cen, epochs, (int)nclusters, threads, assign, user);
+        if (krc != BV_OK) return krc;
         int maxid = 0;
         for (const Pt &p : pts) maxid = std::max(maxid, p.cluster);
         clusterNum.assign((size_t)maxid + 1, 0);

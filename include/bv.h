@@ -213,6 +213,17 @@ int bv_partition(int64_t n, const double *x, const double *y, const double *z,
                  int32_t kmeans_epochs, int32_t time_flag, int32_t distance_metric,
                  int32_t threads, bv_blocks *out);
 
+/* bv_partition with the k-means assignment step (every point against every
+ * centroid, each epoch) on GPU `device`; everything else as bv_partition, on
+ * the host.  The output is bv_partition's, bit for bit: the device uses the
+ * host's rounding and tie rule.  No counterpart in the reference, whose
+ * k-means runs on the host (src/kmeans.cpp:59-145); for large n it dominates
+ * the set-up. */
+int bv_partition_gpu(int32_t device, int64_t n, const double *x, const double *y,
+                     const double *z, const double *obs, int32_t bc, int32_t cs, int32_t seed,
+                     int32_t kmeans_epochs, int32_t time_flag, int32_t distance_metric,
+                     int32_t threads, bv_blocks *out);
+
 void bv_blocks_free(bv_blocks *b);
 
 /* Checks that this build's matrix-core lane layout assumption holds on the
