@@ -199,8 +199,8 @@ def main():
         raise SystemExit("evaluation failed: status %d block %d" % (st, bad))
     cfg = eng.factor_config()
     if cfg is None or cfg[1] == 0:
-        kname = "bv_block_kernel (nu=1/2 closed form, %s wavefront%s per block)" % (
-            "one" if cfg and cfg[0] == 1 else "four", "" if cfg and cfg[0] == 1 else "s")
+        nw = {0: "four wavefronts", 1: "one wavefront", 2: "two wavefronts"}[cfg[0] if cfg else 0]
+        kname = "bv_block_kernel (nu=1/2 closed form, %s per block)" % nw
     else:
         kname = "bv_step_kernel x %d panel-step launches (nu=1/2 closed form, %d chunk(s)%s)" % (
             eng_pmax(blocks, cs), cfg[2], ", HIP graph replay" if cfg[4] else "")

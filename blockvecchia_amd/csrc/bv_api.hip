@@ -23,6 +23,8 @@
 
 hipError_t bv_launch_block(int kind, unsigned nblk, size_t lds, hipStream_t st,
                            const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
+hipError_t bv_launch_block_2w(int kind, unsigned nblk, size_t lds, hipStream_t st,
+                              const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
 hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t st,
                               const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
 hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,
@@ -85,7 +87,7 @@ struct bv_ctx {
     int last_k = 0;
     double last_ms = 0.0;
     int iterations = 0;
-    int variant = 0; /* factorization kernel: 0 = four wavefronts per block, 1 = one */
+    int variant = 0; /* factorization kernel: wavefronts per block 0: four, 1: one, 2: two */
     int stepped = 0; /* 1: one launch per panel step over all blocks */
     /* stepped factorization: work items of every step for desc_evals evaluations */
     bv_step_item *d_items = nullptr;
@@ -305,7 +307,8 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     c->dist_scale = p->dist_scale > 0 ? p->dist_scale : 1.0;
     c->verbose = p->verbose;
     /* factorization kernel: BV_FACTOR_KERNEL=1w selects the one-wavefront one */
-    if (const char *v = getenv("BV_FACTOR_KERNEL")) c->variant = (strcmp(v, "1w") == 0) ? 1 : 0;
+    if (const char *v = getenv("BV_FACTOR_KERNEL"))
+        c->variant = (strcmp(v, "1w") == 0) ? 1 : (strcmp(v, "2w") == 0) ? 2 : 0;
     if (const char *v = getenv("BV_PATH")) c->stepped = (strcmp(v, "stepped") == 0) ? 1 : 0;
     if (const char *v = getenv("BV_CHUNKS")) c->nchunks = std::max(1, std::min(64, atoi(v)));
     if (const char *v = getenv("BV_CHUNK_STREAMS")) c->nstreams = std::max(1, std::min(4, atoi(v)));
@@ -432,6 +435,8 @@ static int bv_enqueue(bv_ctx *c, unsigned nblk, int kinds_used) {
                                         c->pmax));
         } else if (c->variant == 1)
             BV_HIP(bv_launch_block_1w(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
+        else if (c->variant == 2)
+            BV_HIP(bv_launch_block_2w(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
         else
             BV_HIP(bv_launch_block(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
     }
@@ -556,13 +561,14 @@ static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, doubl
 }
 
 /* Factorization configurations the autotuner chooses from: the fused kernel
-   with four wavefronts or one wavefront per block, and the stepped
+   with four, one or two wavefronts per block, and the stepped
    factorization, whole or in two chunks over two streams.  Which is fastest
    depends on the block sizes and the covariance kind. */
 struct bv_cfg {
     int variant, stepped, nchunks, nstreams;
 };
-static const bv_cfg bv_cfgs[] = {{0, 0, 1, 1}, {1, 0, 1, 1}, {0, 1, 1, 1}, {0, 1, 2, 2}};
+static const bv_cfg bv_cfgs[] = {{0, 0, 1, 1}, {1, 0, 1, 1}, {2, 0, 1, 1}, {0, 1, 1, 1},
+                                 {0, 1, 2, 2}};
 
 static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
     const bool relayout = g.nchunks != c->nchunks;

@@ -116,10 +116,10 @@ __device__ __forceinline__ void bv_gemm_load(const double *colp, int rt0, int rb
    issue, with no register copies between iterations.  Tile (rt, kt) sits at
    ws + (kt T - kt(kt-1)/2 + rt - kt) * 256: the column part is wave-uniform and
    advances by T - kt - 1 tiles per column, the lane part is 4 * lane. */
-template <int C>
+template <int C, int N>
 __device__ __forceinline__ void bv_panel_gemm(const double *__restrict__ ws, int T, int j,
                                               int rbase, int rstride, int lane,
-                                              bv_d4 (&acc)[BV_NRT][2]) {
+                                              bv_d4 (&acc)[N][2]) {
     const int rt0 = 2 * j;
     if (rt0 == 0) return;
     const int lo = 4 * lane;
@@ -130,11 +130,11 @@ __device__ __forceinline__ void bv_panel_gemm(const double *__restrict__ ws, int
         /* column kt+1 (rt0 is even, so it always exists here) */
         const double *colb = colp + ((int64_t)(T - kt - 1) << 8);
         bv_gemm_load<C>(colb, rt0, rbase, rstride, lo, jb0, jb1, rb);
-        bv_gemm_step<C, BV_NRT>(ja0, ja1, ra, acc);
+        bv_gemm_step<C, N>(ja0, ja1, ra, acc);
         /* column kt+2, or column kt+1 again after the last pair */
         const double *cola = (kt + 2 < rt0) ? colb + ((int64_t)(T - kt - 2) << 8) : colb;
         bv_gemm_load<C>(cola, rt0, rbase, rstride, lo, ja0, ja1, ra);
-        bv_gemm_step<C, BV_NRT>(jb0, jb1, rb, acc);
+        bv_gemm_step<C, N>(jb0, jb1, rb, acc);
         colp = cola;
     }
 }
@@ -183,10 +183,10 @@ __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T
 
 /* Phase 1 of a panel for C row tiles: acc = A - products, stored as C into
    the tiles (rt, rt0) and (rt, rt0+1) (the slots L will take). */
-template <int C>
+template <int C, int N>
 __device__ __forceinline__ void bv_pass_compute_c(double *__restrict__ ws, int T, int j, int rbase,
                                                   int rstride, int lane) {
-    bv_d4 acc[BV_NRT][2];
+    bv_d4 acc[N][2];
     const int rt0 = 2 * j;
     /* acc = -A; the products are added; C = -acc (negations are exact) */
 #pragma unroll
@@ -196,7 +196,7 @@ __device__ __forceinline__ void bv_pass_compute_c(double *__restrict__ ws, int T
         acc[s][1] = (rt > rt0) ? -*(const bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane)
                                : (bv_d4){0.0, 0.0, 0.0, 0.0};
     }
-    bv_panel_gemm<C>(ws, T, j, rbase, rstride, lane, acc);
+    bv_panel_gemm<C, N>(ws, T, j, rbase, rstride, lane, acc);
 #pragma unroll
     for (int s = 0; s < C; ++s) {
         const int rt = rbase + rstride * s;
@@ -205,19 +205,15 @@ __device__ __forceinline__ void bv_pass_compute_c(double *__restrict__ ws, int T
     }
 }
 
+/* c row tiles (c <= N, the most a pass holds) */
+template <int N>
 __device__ __forceinline__ void bv_pass_compute(int c, double *__restrict__ ws, int T, int j,
                                                 int rbase, int rstride, int lane) {
     switch (c) {
-    case 1: bv_pass_compute_c<1>(ws, T, j, rbase, rstride, lane); break;
-#if BV_NRT >= 2
-    case 2: bv_pass_compute_c<2>(ws, T, j, rbase, rstride, lane); break;
-#endif
-#if BV_NRT >= 3
-    case 3: bv_pass_compute_c<3>(ws, T, j, rbase, rstride, lane); break;
-#endif
-#if BV_NRT >= 4
-    case 4: bv_pass_compute_c<4>(ws, T, j, rbase, rstride, lane); break;
-#endif
+    case 1: bv_pass_compute_c<1, N>(ws, T, j, rbase, rstride, lane); break;
+    case 2: if (N >= 2) bv_pass_compute_c<(N >= 2 ? 2 : 1), N>(ws, T, j, rbase, rstride, lane); break;
+    case 3: if (N >= 3) bv_pass_compute_c<(N >= 3 ? 3 : 1), N>(ws, T, j, rbase, rstride, lane); break;
+    case 4: if (N >= 4) bv_pass_compute_c<(N >= 4 ? 4 : 1), N>(ws, T, j, rbase, rstride, lane); break;
     default: break;
     }
 }
@@ -428,15 +424,23 @@ __device__ __forceinline__ void bv_diag_factor_wave(const double *__restrict__ w
                          sT, sW, sPiv, s_dotd, s_info);
 }
 
-template <int KIND>
-__global__ void __launch_bounds__(BV_THREADS, BV_WAVES_PER_SIMD)
+/* NW wavefronts per block: 4 (BV_WAVES_PER_SIMD per SIMD) or 2
+   (BV_2W_WAVES_PER_SIMD: fewer registers, so more blocks resident at once). */
+#ifndef BV_2W_WAVES_PER_SIMD
+#define BV_2W_WAVES_PER_SIMD 3
+#endif
+#ifndef BV_2W_NRT
+#define BV_2W_NRT 2
+#endif
+template <int KIND, int NW>
+__global__ void __launch_bounds__(64 * NW, NW == 4 ? BV_WAVES_PER_SIMD : BV_2W_WAVES_PER_SIMD)
 bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
                 double *ws_base) {
     extern __shared__ double smem[];
     __shared__ double sW[3 * 256];
     __shared__ double sT[16 * 17];
     __shared__ double sCol[16], sRow[16], sPiv[BV_NB];
-    __shared__ double s_logdet, s_dotd, s_wdot[BV_THREADS / 64];
+    __shared__ double s_logdet, s_dotd, s_wdot[NW];
     __shared__ int s_info;
     __shared__ bv_kparams s_kp; /* read where used: not held in registers */
 
@@ -448,10 +452,11 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
     const int T = Mp / BV_TILE, P = Mp / BV_NB;
     double *sx = smem, *sy = sx + Mp, *sz = sy + Mp, *so = sz + Mp;
     const bv_kparams &kp = s_kp;
+    constexpr int NRT = (NW == 4) ? BV_NRT : BV_2W_NRT; /* row tiles per GEMM pass */
 
     if (KIND != BV_KIND_LOADA) {
         /* stage the block's points: conditioning set first, then the block */
-        for (int p = tid0; p < Mp; p += BV_THREADS) {
+        for (int p = tid0; p < Mp; p += 64 * NW) {
             double xv, yv, zv, ov;
             bv_point(d, E, p, xv, yv, zv, ov);
             sx[p] = xv; sy[p] = yv; sz[p] = zv; so[p] = ov;
@@ -473,27 +478,27 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
         const int J0 = j * BV_NB;
         const int rt0 = 2 * j;
         const int nrt = T - rt0;
-        /* phase 1: row tiles rt0 + wave + 4i, in passes of at most BV_NRT */
-        const int cnt = (nrt > wave) ? (nrt - wave + 3) / 4 : 0;
-        const int ngrp = (cnt + BV_NRT - 1) / BV_NRT;
+        /* phase 1: row tiles rt0 + wave + NW i, in passes of at most NRT */
+        const int cnt = (nrt > wave) ? (nrt - wave + NW - 1) / NW : 0;
+        const int ngrp = (cnt + NRT - 1) / NRT;
         const int gsz = ngrp ? (cnt + ngrp - 1) / ngrp : 0;
         /* phase 0 for the first panel (later panels are generated during the
            previous panel's phase 2) */
         if (KIND != BV_KIND_LOADA && j == 0) {
-            bv_panel_generate<KIND>(ws, T, 0, wave, 4, lane, m, sx, sy, sz, so, kp);
+            bv_panel_generate<KIND>(ws, T, 0, wave, NW, lane, m, sx, sy, sz, so, kp);
             __syncthreads();
         }
         for (int gi = 0; gi < ngrp; ++gi) {
             const int i0 = gi * gsz;
             const int c = (cnt - i0) < gsz ? (cnt - i0) : gsz;
-            bv_pass_compute(c, ws, T, j, rt0 + wave + 4 * i0, 4, lane);
+            bv_pass_compute<NRT>(c, ws, T, j, rt0 + wave + NW * i0, NW, lane);
         }
         __syncthreads();
         /* phase 2 (wavefront 0): factor and invert the diagonal block, while
-           wavefronts 1-3 generate the next panel's covariance tiles */
+           the other wavefronts generate the next panel's covariance tiles */
         if (KIND != BV_KIND_LOADA && wave > 0 && j + 1 < P)
-            bv_panel_generate<KIND>(ws, T, j + 1, rt0 + 2 + wave - 1, 3, lane, m, sx, sy, sz, so,
-                                    kp);
+            bv_panel_generate<KIND>(ws, T, j + 1, rt0 + 2 + wave - 1, NW - 1, lane, m, sx, sy, sz,
+                                    so, kp);
         if (wave == 0) {
             bv_diag_factor_wave(ws, T, rt0, lane, J0, m, cs0, sCol, sRow, sT, sW, sPiv, &s_dotd,
                                 &s_info);
@@ -505,7 +510,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
             const bv_d4 w00 = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
             const bv_d4 w10 = *(const bv_d4 *)(sW + 1 * 256 + 4 * lane);
             const bv_d4 w11 = *(const bv_d4 *)(sW + 2 * 256 + 4 * lane);
-            for (int rt = rt0 + 2 + wave; rt < T; rt += 4)
+            for (int rt = rt0 + 2 + wave; rt < T; rt += NW)
                 bv_tile_solve(ws, T, j, rt, lane, m, cs0, w00, w10, w11, dot_lane);
         }
         __syncthreads();
@@ -518,7 +523,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
     __syncthreads();
     if (tid0 == 0) {
         double dsum = s_dotd;
-        for (int w = 0; w < BV_THREADS / 64; ++w) dsum += s_wdot[w];
+        for (int w = 0; w < NW; ++w) dsum += s_wdot[w];
         E.logdet[d.block] = s_logdet;
         E.dot[d.block] = dsum;
         E.info[d.block] = s_info;
@@ -580,7 +585,7 @@ bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *_
         for (int gi = 0; gi < ngrp; ++gi) {
             const int i0 = gi * gsz;
             const int c = (nrt - i0) < gsz ? (nrt - i0) : gsz;
-            bv_pass_compute(c, ws, T, j, rt0 + i0, 1, lane);
+            bv_pass_compute<BV_NRT>(c, ws, T, j, rt0 + i0, 1, lane);
         }
         bv_diag_factor_wave(ws, T, rt0, lane, J0, m, cs0, sCol, sRow, sT, sW, sPiv, &s_dotd,
                             &s_info);
@@ -1104,27 +1109,40 @@ hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t bv_launch_block(int kind, unsigned nblk, size_t lds, hipStream_t st,
-                           const bv_block_desc *desc, const bv_eval_slot *slots, double *ws) {
+template <int NW>
+static hipError_t bv_launch_block_nw(int kind, unsigned nblk, size_t lds, hipStream_t st,
+                                     const bv_block_desc *desc, const bv_eval_slot *slots,
+                                     double *ws) {
+    const dim3 blk(64 * NW);
     switch (kind) {
     case BV_KIND_NU05:
-        hipLaunchKernelGGL(bv_block_kernel<BV_KIND_NU05>, dim3(nblk), dim3(BV_THREADS), lds, st,
-                           desc, slots, ws);
+        hipLaunchKernelGGL((bv_block_kernel<BV_KIND_NU05, NW>), dim3(nblk), blk, lds, st, desc,
+                           slots, ws);
         break;
     case BV_KIND_NU15:
-        hipLaunchKernelGGL(bv_block_kernel<BV_KIND_NU15>, dim3(nblk), dim3(BV_THREADS), lds, st,
-                           desc, slots, ws);
+        hipLaunchKernelGGL((bv_block_kernel<BV_KIND_NU15, NW>), dim3(nblk), blk, lds, st, desc,
+                           slots, ws);
         break;
     case BV_KIND_NU25:
-        hipLaunchKernelGGL(bv_block_kernel<BV_KIND_NU25>, dim3(nblk), dim3(BV_THREADS), lds, st,
-                           desc, slots, ws);
+        hipLaunchKernelGGL((bv_block_kernel<BV_KIND_NU25, NW>), dim3(nblk), blk, lds, st, desc,
+                           slots, ws);
         break;
     default:
-        hipLaunchKernelGGL(bv_block_kernel<BV_KIND_LOADA>, dim3(nblk), dim3(BV_THREADS), 0, st,
-                           desc, slots, ws);
+        hipLaunchKernelGGL((bv_block_kernel<BV_KIND_LOADA, NW>), dim3(nblk), blk, 0, st, desc,
+                           slots, ws);
         break;
     }
     return hipGetLastError();
+}
+
+hipError_t bv_launch_block(int kind, unsigned nblk, size_t lds, hipStream_t st,
+                           const bv_block_desc *desc, const bv_eval_slot *slots, double *ws) {
+    return bv_launch_block_nw<4>(kind, nblk, lds, st, desc, slots, ws);
+}
+
+hipError_t bv_launch_block_2w(int kind, unsigned nblk, size_t lds, hipStream_t st,
+                              const bv_block_desc *desc, const bv_eval_slot *slots, double *ws) {
+    return bv_launch_block_nw<2>(kind, nblk, lds, st, desc, slots, ws);
 }
 
 hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,

@@ -109,7 +109,7 @@ int bv_last_status(bv_ctx *ctx, int *first_bad_block);
 double bv_last_kernel_ms(bv_ctx *ctx);
 
 /* The factorization configuration in use: cfg[0] = kernel variant (0: four
- * wavefronts per block, 1: one), cfg[1] = 1 for the stepped factorization
+ * wavefronts per block, 1: one, 2: two), cfg[1] = 1 for the stepped factorization
  * (one launch per panel step over all blocks), cfg[2] = block chunks,
  * cfg[3] = streams, cfg[4] = 1 when single evaluations replay a HIP graph
  * (cfg holds 5 entries).  No counterpart in the reference.  The first

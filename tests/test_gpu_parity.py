@@ -271,19 +271,28 @@ def test_n20000_scaling_law(c2_blocks):
         eng.close()
 
 
-@pytest.mark.parametrize("chunks,streams", [(1, 1), (3, 2)])
-def test_stepped_path_matches_oracle(c1_full, chunks, streams, monkeypatch):
-    """The stepped factorization (one launch per panel step over all blocks,
-    BV_PATH=stepped, optionally chunked over streams) against the oracle: the
-    n=2000 workload, a general-nu case and a large-block case."""
-    monkeypatch.setenv("BV_PATH", "stepped")
-    monkeypatch.setenv("BV_CHUNKS", str(chunks))
-    monkeypatch.setenv("BV_CHUNK_STREAMS", str(streams))
-    _say("case: stepped chunks=%d streams=%d n=2000" % (chunks, streams))
+FACTOR_CONFIGS = {
+    "fused_4w": {"BV_PATH": "fused"},
+    "fused_1w": {"BV_FACTOR_KERNEL": "1w"},
+    "fused_2w": {"BV_FACTOR_KERNEL": "2w"},
+    "stepped": {"BV_PATH": "stepped", "BV_CHUNKS": "1", "BV_CHUNK_STREAMS": "1"},
+    "stepped_3chunks_2streams": {"BV_PATH": "stepped", "BV_CHUNKS": "3", "BV_CHUNK_STREAMS": "2"},
+}
+
+
+@pytest.mark.parametrize("name", sorted(FACTOR_CONFIGS))
+def test_factor_configs_match_oracle(c1_full, name, monkeypatch):
+    """Every factorization configuration the autotuner can choose, fixed by
+    environment (the fused kernel with four, one or two wavefronts per block;
+    the stepped factorization, whole or chunked over streams) against the
+    oracle: the n=2000 workload, a general-nu case and a large-block case."""
+    for k, v in FACTOR_CONFIGS[name].items():
+        monkeypatch.setenv(k, v)
+    _say("case: %s n=2000" % name)
     x, y, obs = c1_full
     _check(bv.partition(x, y, obs, 100, 60, seed=0), [1.5, 0.1, 0.5])
     x, y = datagen.uniform_locations(600, 0)
     obs = datagen.sample_dense(x, y, 1.0, 0.1, 0.5, seed=0)
     for bc, cs, th in [(30, 40, [1.0, 0.08, 0.8]), (12, 250, [1.0, 0.1, 0.5])]:
-        _say("case: stepped n=600 bc=%d cs=%d theta=%s" % (bc, cs, th))
+        _say("case: %s n=600 bc=%d cs=%d theta=%s" % (name, bc, cs, th))
         _check(bv.partition(x, y, obs, bc, cs, seed=0), th)
