@@ -156,6 +156,130 @@ def eng_pmax(blocks, cs):
     return ((m + 1 + 31) // 32 * 32) // 32
 
 
+def _with_obs(blocks, obs):
+    import copy
+    b = copy.copy(blocks)
+    b.obs = np.ascontiguousarray(np.asarray(obs), dtype=np.float64)
+    return b
+
+
+def run_uneven(args):
+    """Uneven batching: n = 200,000 points, 70% uniform and 30% from 20
+    Gaussian clusters (sd 0.02), k-means into 10,000 blocks (1 to ~100
+    points each), cs = 400; Matern sigma^2 = 1, range 0.05, nu in {3/2, 5/2};
+    observations from the Block Vecchia model of the partition.  One JSON
+    line: evaluations/s and the factorization's FP64 rate per nu."""
+    import blockvecchia_amd as bv
+    from blockvecchia_amd import datagen, flops
+
+    n, bc, cs = 200000, 10000, 400
+    x, y = datagen.clustered_locations(n, 0)
+    t0 = time.time()
+    blocks = bv.partition(x, y, np.zeros(n), bc, cs, seed=0, device=0)
+    bn = blocks.batch_num
+    log("partition (GPU k-means, host kNN): %.1f s, %d blocks, block sizes %d..%d (block 0: %d)"
+        % (time.time() - t0, blocks.batch_count, bn[1:].min(), bn[1:].max(), bn[0]))
+    F = flops.eval_flops(bn, cs)
+    out = []
+    for nu in (1.5, 2.5):
+        theta = [1.0, 0.05, nu]
+        t0 = time.time()
+        obs = datagen.sample_block_vecchia(blocks, *theta, seed=0, threads=16)
+        log("nu=%g observations (Block Vecchia simulation): %.1f s, %d jittered factorizations"
+            % (nu, time.time() - t0, obs.blocks_jittered))
+        eng = bv.Engine(_with_obs(blocks, obs))
+        for _ in range(args.warmup):
+            eng.llh(theta)
+        lat, kms = [], []
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            v = eng.llh(theta)
+            lat.append(time.perf_counter() - t1)
+            kms.append(eng.last_kernel_ms())
+        el = time.perf_counter() - t_start
+        info = eng.block_terms()[2]
+        kern = float(np.mean(kms))
+        out.append({
+            "nu": nu, "evals_per_s": args.steps / el, "ms_per_eval": 1e3 * el / args.steps,
+            "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                           "p99": float(np.percentile(lat, 99) * 1e3)},
+            "llh": v, "blocks_not_positive_definite": int(np.count_nonzero(info)),
+            "factor_config": list(eng.factor_config()),
+            "roofline": {"bound": "mfma", "achieved": F / (kern * 1e-3) / 1e12,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": F / (kern * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                         "kernel_ms": kern, "flops_per_launch": F},
+        })
+        log("nu=%g: %.3f ms/eval, %.2f TFLOP/s, %d non-PD blocks" % (
+            nu, out[-1]["ms_per_eval"], out[-1]["roofline"]["achieved"],
+            out[-1]["blocks_not_positive_definite"]))
+        eng.close()
+    print(json.dumps({
+        "metric": METRIC, "value": out[0]["evals_per_s"], "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": out[0]["ms_per_eval"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "uneven batching: n=200000 (70% uniform, 30% in 20 clusters sd "
+                               "0.02), k-means bc=10000, cs=400, sigma2=1 range=0.05; value at "
+                               "nu=3/2", "n": n, "bc": bc, "cs": cs,
+                   "batch_count": int(blocks.batch_count), "parallelism": "single-gpu"},
+        "per_nu": out}))
+
+
+def run_mle(args):
+    """Full maximum-likelihood run with general smoothness: n = 100,000
+    uniform, truth (1.0, 0.08, 0.8), observations from the Block Vecchia
+    model of the k-means partition (bc = 5,000, cs = 150); the driver
+    (blockvecchia_amd/bin/test_dvecchia_batch) maximises from 0.1:0.1:0.1
+    with xtol 1e-4.  One JSON line: wall-clock to convergence (lower is
+    better), evaluations, estimates and the final log-likelihood."""
+    import subprocess
+    import tempfile
+
+    import blockvecchia_amd as bv
+    from blockvecchia_amd import datagen
+
+    n, bc, cs = 100000, 5000, 150
+    theta = [1.0, 0.08, 0.8]
+    x, y = datagen.uniform_locations(n, 0)
+    blocks = bv.partition(x, y, np.zeros(n), bc, cs, seed=0, device=0)
+    t0 = time.time()
+    obs_b = datagen.sample_block_vecchia(blocks, *theta, seed=0, threads=16)
+    obs = np.empty(n)
+    obs[np.asarray(blocks.perm)] = np.asarray(obs_b)
+    log("observations (Block Vecchia simulation): %.1f s" % (time.time() - t0))
+    driver = os.path.join(ROOT, "blockvecchia_amd", "bin", "test_dvecchia_batch")
+    with tempfile.TemporaryDirectory() as tmp:
+        datagen.write_xy_csv(os.path.join(tmp, "xy.csv"), x, y)
+        datagen.write_obs_csv(os.path.join(tmp, "obs.csv"), obs)
+        cmd = [driver, "--xy_path", os.path.join(tmp, "xy.csv"), "--obs_path",
+               os.path.join(tmp, "obs.csv"), "--num_loc", str(n), "--vecchia_bc", str(bc),
+               "--vecchia_cs", str(cs), "--kernel_init", "0.1:0.1:0.1", "--ikernel", "?:?:?",
+               "--tol", "4", "--omp_threads", "16", "--seed", "0"]
+        log("driver: " + " ".join(cmd))
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True, timeout=900)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise SystemExit("driver failed (%d): %s" % (r.returncode, r.stderr[-2000:]))
+        logs = os.listdir(os.path.join(tmp, "log"))
+        vals = [v.strip() for v in
+                open(os.path.join(tmp, "log", logs[0])).read().strip().splitlines()[1].split(",")]
+    iters, est, llh, opt_s = int(vals[0]), [float(v) for v in vals[1:4]], float(vals[4]), \
+        float(vals[5])
+    print(json.dumps({
+        "metric": "MLE wall-clock to convergence (driver, general nu)", "value": opt_s,
+        "unit": "s", "n_gpus": 1, "steps": iters, "warmup": 0, "ms_per_step": 1e3 * opt_s / iters,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "MLE: n=100000 uniform, truth (1.0, 0.08, 0.8), Block Vecchia "
+                               "observations, bc=5000, cs=150, init 0.1:0.1:0.1, xtol 1e-4",
+                   "n": n, "bc": bc, "cs": cs, "parallelism": "single-gpu"},
+        "evaluations": iters, "estimate": est, "truth": theta, "llh": llh,
+        "driver_wall_s": wall}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +293,10 @@ def main():
                     help="append a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 counter CSVs into the traffic JSON and exit")
+    ap.add_argument("--workload", choices=["perf", "uneven", "mle"], default="perf",
+                    help="perf: the n=20000 cs=300 evaluation (default); uneven: n=200000 "
+                         "clustered, cs=400, nu 3/2 and 5/2; mle: the driver's full "
+                         "maximum-likelihood run at n=100000, general nu")
     ap.add_argument("--config", default=None,
                     help="with --traffic-from: the factorization configuration the counters "
                          "were taken with, as variant,stepped,chunks,streams")
@@ -183,6 +311,10 @@ def main():
               "across GPUs with a single 8-byte sum; see DESIGN.md); --gpus %d is not run"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "uneven":
+        return run_uneven(args)
+    if args.workload == "mle":
+        return run_mle(args)
 
     import blockvecchia_amd as bv
     from blockvecchia_amd import flops

@@ -123,7 +123,14 @@ inline Result maximize(const std::function<double(const std::vector<double> &)> 
     /* the structured design around c with step h (h <= span / 3): c,
        c + d_k e_k, c - d_k e_k (or c + 2 d_k e_k at a bound), and
        c + d_a e_a + d_b e_b; poised for the full quadratic by construction */
-    auto rebuild = [&](const std::vector<double> &c, double fc, double h) {
+    /* centre and step of the last rebuild: rebuilding the same design again
+       would reproduce the same model and the same rejected step */
+    std::vector<double> last_c;
+    double last_h = -1.0;
+    auto rebuild = [&](const std::vector<double> c, double fc, double h) {
+        /* c by value: callers pass points of Y, which the next line replaces */
+        last_c = c;
+        last_h = h;
         Y.assign(1, c);
         FY.assign(1, fc);
         std::vector<double> d(n);
@@ -160,6 +167,12 @@ inline Result maximize(const std::function<double(const std::vector<double> &)> 
         double s = 0.0;
         for (int k = 0; k < n; ++k) s += (a[k] - b[k]) * (a[k] - b[k]);
         return sqrt(s);
+    };
+    /* the set is spread beyond 2 r around c (the design itself reaches 2 h
+       next to a bound), and rebuilding around c would not repeat the last
+       rebuild */
+    auto spread_out = [&](const std::vector<double> &c, double fard, double r) {
+        return fard > 2.0 * r * (1.0 + 1e-9) && !(last_h == rho && last_c == c);
     };
     auto project = [&](const std::vector<double> &c, std::vector<double> &s) {
         for (int k = 0; k < n; ++k) s[k] = std::min(std::max(s[k], zl[k] - c[k]), zu[k] - c[k]);
@@ -200,7 +213,14 @@ inline Result maximize(const std::function<double(const std::vector<double> &)> 
         }
         if (ok) ok = solve_dense(M, rhs, p, 1e-13);
         if (!ok) {
-            /* poorly poised, incomplete, or a failed evaluation in the set */
+            /* poorly poised, incomplete, or a failed evaluation in the set;
+               when the design just built here gives no model (failed
+               evaluations in it), a smaller one */
+            if (last_h == rho && last_c == c) {
+                rho = std::max(0.1 * rho, 0.5 * rho_end);
+                delta = std::max(0.5 * delta, rho);
+                if (rho <= rho_end) { res.status = BVOPT_XTOL; break; }
+            }
             rebuild(c, fc, rho);
             continue;
         }
@@ -258,7 +278,7 @@ inline Result maximize(const std::function<double(const std::vector<double> &)> 
         const double sn2 = sqrt(std::inner_product(best_s.begin(), best_s.end(), best_s.begin(), 0.0));
         if (sn2 < 0.5 * rho || !(qs < 0.0)) {
             /* no useful step at this radius: repair geometry, else shrink rho */
-            if (far >= 0 && fard > 2.0 * rho) {
+            if (far >= 0 && spread_out(c, fard, rho)) {
                 rebuild(c, fc, rho);
             } else {
                 rho = std::max(0.1 * rho, 0.5 * rho_end);
@@ -287,7 +307,10 @@ inline Result maximize(const std::function<double(const std::vector<double> &)> 
             FY[j] = fz;
         }
         if (ratio < 0.1 && res.evals < maxeval) {
-            if (far >= 0 && fard > 2.0 * delta) rebuild(Y[best_index()], FY[best_index()], rho);
+            const int b2 = best_index();
+            if (far >= 0 && spread_out(Y[b2], fard, delta)) {
+                rebuild(Y[b2], FY[b2], rho);
+            }
             else if (delta <= rho) {
                 rho = std::max(0.1 * rho, 0.5 * rho_end);
                 delta = std::max(0.5 * delta, rho);

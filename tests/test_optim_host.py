@@ -132,3 +132,57 @@ def test_vecchia_mle_agrees_with_lbfgsb(shim):
                    bounds=[(0.01, 3.0), (0.01, 3.0)], options={"ftol": 1e-14, "gtol": 1e-9})
     assert abs(fo - (-ref.fun)) <= 1e-8 * abs(ref.fun), (fo, -ref.fun)
     assert np.allclose(xo[:2], ref.x, rtol=2e-4), (xo, ref.x)
+
+
+def _recorded(f, lb, ub):
+    pts = []
+
+    def g(x):
+        pts.append(x.copy())
+        return f(x)
+    return g, pts
+
+
+@pytest.mark.parametrize("start", [[0.1, 0.1, 0.1], [2.9, 0.05, 1.7], [0.3, 2.5, 0.02]])
+def test_every_evaluated_point_in_bounds(shim, start):
+    """Rebuilding the interpolation set after a rejected step must start from
+    a copy of the best point (the set it is taken from is overwritten):
+    every evaluated point stays finite and inside the box, on a curved
+    valley that forces rejected steps and rebuilds."""
+    lb, ub = [0.01] * 3, [3.0] * 3
+
+    def valley(x):
+        # a banana in (x0, x1) with a steep wall in x2
+        return -(50.0 * (x[1] - x[0] ** 2) ** 2 + (1.0 - x[0]) ** 2 + 20.0 * (x[2] - 0.8) ** 2
+                 + 5.0 * np.sin(7.0 * x[2]) ** 2)
+    g, pts = _recorded(valley, lb, ub)
+    x, fx, ev, st = maximize(shim, g, start, lb, ub, xtol=1e-6, maxeval=1500)
+    P = np.array(pts)
+    assert np.isfinite(P).all()
+    assert (P >= np.array(lb)).all() and (P <= np.array(ub)).all(), P[(P < lb).any(1) | (P > ub).any(1)][:3]
+    assert ev == len(pts)
+
+
+def test_general_nu_mle_from_default_start(shim):
+    """All three parameters free, general smoothness, from the reference's
+    default start 0.1:0.1:0.1 (n = 300, 23 blocks, cs = 30, observations
+    drawn from the Block Vecchia model at (1.0, 0.08, 0.8)).  The maximum of
+    the oracle's likelihood, found by scipy's Nelder-Mead (300 evaluations)
+    and Powell (1306) from the same start, both to 1e-7 in x:
+    (0.70817772, 0.06900015, 0.78882709), llh = -207.94945712489.  This start
+    once made the optimizer cycle on the initial lattice (a rejected step,
+    then the same design rebuilt at the same radius, until maxeval)."""
+    import blockvecchia_amd as bv
+    from blockvecchia_amd import datagen
+    import oracle
+
+    n = 300
+    x, y = datagen.uniform_locations(n, 0)
+    b = bv.partition(x, y, np.zeros(n), n // 13, 30, seed=0)
+    b.obs = np.ascontiguousarray(np.asarray(datagen.sample_block_vecchia(b, 1.0, 0.08, 0.8, seed=0)))
+    f = lambda t: oracle.llh(b, [t[0], t[1], t[2]])
+    xo, fo, ev, st = maximize(shim, f, [0.1, 0.1, 0.1], [0.01] * 3, [3.0] * 3, xtol=1e-7,
+                              maxeval=1000)
+    assert st == 1 and ev < 1000, (st, ev, xo)
+    assert abs(fo - (-207.94945712489)) <= 1e-9 * 207.95, fo
+    assert np.allclose(xo, [0.70817772, 0.06900015, 0.78882709], rtol=2e-4), xo
