@@ -414,11 +414,16 @@ static int bv_enqueue(bv_ctx *c, unsigned nblk, int kinds_used) {
                 for (int i = 0; i < c->nstreams; ++i)
                     BV_HIP(hipStreamWaitEvent(c->cstream[i], c->cev[c->nstreams], 0));
             }
-            for (int ch = 0; ch < c->nchunks; ++ch) {
-                hipStream_t st = side ? c->cstream[ch % c->nstreams] : c->stream;
-                const int64_t *off = c->step_off.data() + (size_t)ch * (c->pmax + 1);
-                for (int j = 0; j < c->pmax; ++j) {
+            /* step j of every chunk before step j + 1 of any: each stream
+               gets its first launch at once, so the chunks' chains overlap
+               from the start (on one stream, chunk by chunk was slower: the
+               launches lose parallelism and gain no cache reuse) */
+            for (int j = 0; j < c->pmax; ++j) {
+                for (int ch = 0; ch < c->nchunks; ++ch) {
+                    hipStream_t st = side ? c->cstream[ch % c->nstreams] : c->stream;
+                    const int64_t *off = c->step_off.data() + (size_t)ch * (c->pmax + 1);
                     const int64_t o = off[j], cnt = off[j + 1] - o;
+                    if (cnt <= 0) continue;
                     BV_HIP(bv_launch_step(kind, (unsigned)cnt, j, st, c->d_items + o, c->d_desc,
                                           c->d_slots, c->d_ws, c->d_part_logdet, c->d_part_dot,
                                           c->d_part_info, c->pmax));
@@ -562,13 +567,13 @@ static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, doubl
 
 /* Factorization configurations the autotuner chooses from: the fused kernel
    with four, one or two wavefronts per block, and the stepped
-   factorization, whole or in two chunks over two streams.  Which is fastest
+   factorization, whole or in two or three chunks over as many streams.  Which is fastest
    depends on the block sizes and the covariance kind. */
 struct bv_cfg {
     int variant, stepped, nchunks, nstreams;
 };
 static const bv_cfg bv_cfgs[] = {{0, 0, 1, 1}, {1, 0, 1, 1}, {2, 0, 1, 1}, {0, 1, 1, 1},
-                                 {0, 1, 2, 2}};
+                                 {0, 1, 2, 2}, {0, 1, 3, 3}};
 
 static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
     const bool relayout = g.nchunks != c->nchunks;
