@@ -103,7 +103,7 @@ struct bv_ctx {
        allowed (use_graph) and, once tuned, when that timed faster */
     int use_graph = 1, graph_wins = 1;
     hipGraphExec_t graph = nullptr;
-    int64_t graph_key[10] = {};
+    int64_t graph_key[12] = {};
 };
 
 static int bv_nparams(int kernel) {
@@ -461,6 +461,8 @@ static void bv_graph_key(const bv_ctx *c, int kinds_used, int64_t *key) {
     key[7] = c->desc_evals;
     key[8] = c->pmax;
     key[9] = c->max_mp;
+    key[10] = (int64_t)(intptr_t)c->d_part_dot;
+    key[11] = (int64_t)(intptr_t)c->d_part_info;
 }
 
 /* bv_enqueue through a graph captured on first use and replayed while its
@@ -468,7 +470,7 @@ static void bv_graph_key(const bv_ctx *c, int kinds_used, int64_t *key) {
    nodes); the per-call parameters reach it through the slots, copied
    before the launch. */
 static int bv_enqueue_graph(bv_ctx *c, unsigned nblk, int kinds_used) {
-    int64_t key[10];
+    int64_t key[12];
     bv_graph_key(c, kinds_used, key);
     if (c->graph && memcmp(key, c->graph_key, sizeof(key)) != 0) {
         (void)hipGraphExecDestroy(c->graph);
