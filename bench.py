@@ -97,7 +97,8 @@ def calibration_copy(mib, repeats=3):
     torch.cuda.synchronize()
 
 
-FACTOR_PREFIXES = ("void bv_block_kernel<", "void bv_step_kernel<", "bv_step_parts_kernel",
+FACTOR_PREFIXES = ("void bv_block_kernel<", "void bv_step_kernel<", "void bv_step2_kernel<",
+                   "void bv_block_kernel_1w<", "bv_step_parts_kernel",
                    "void bv_gen_kernel<")
 
 
@@ -341,7 +342,8 @@ def main():
         nw = {0: "four wavefronts", 1: "one wavefront", 2: "two wavefronts"}[cfg[0] if cfg else 0]
         kname = "bv_block_kernel (%s, %s per block)" % (nu_desc, nw)
     else:
-        kname = "bv_step_kernel x %d panel-step launches (%s, %d chunk(s)%s)" % (
+        kname = "%s x %d panel-step launches (%s, %d chunk(s)%s)" % (
+            "bv_step2_kernel (lookahead)" if cfg[1] == 2 else "bv_step_kernel",
             eng_pmax(blocks, cs), nu_desc, cfg[2], ", HIP graph replay" if cfg[4] else "")
     log("factorization configuration (autotuned): %s" % (cfg,))
     try:

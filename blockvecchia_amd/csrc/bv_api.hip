@@ -34,6 +34,10 @@ hipError_t bv_launch_step(int kind, unsigned nitems, int j, hipStream_t st,
                           const bv_step_item *items, const bv_block_desc *desc,
                           const bv_eval_slot *slots, double *ws, double *part_logdet,
                           double *part_dot, int32_t *part_info, int pmax);
+hipError_t bv_launch_step2(int kind, unsigned nitems, int j, hipStream_t st,
+                           const bv_step_item *items, const bv_block_desc *desc,
+                           const bv_eval_slot *slots, double *ws, double *part_logdet,
+                           double *part_dot, int32_t *part_info, int pmax);
 hipError_t bv_launch_step_parts(int ndesc, hipStream_t st, const bv_block_desc *desc,
                                 const bv_eval_slot *slots, const double *part_logdet,
                                 const double *part_dot, const int32_t *part_info, int pmax);
@@ -88,7 +92,7 @@ struct bv_ctx {
     double last_ms = 0.0;
     int iterations = 0;
     int variant = 0; /* factorization kernel: wavefronts per block 0: four, 1: one, 2: two */
-    int stepped = 0; /* 1: one launch per panel step over all blocks */
+    int stepped = 0; /* one launch per panel step over all blocks: 1, or 2 with lookahead */
     /* stepped factorization: work items of every step for desc_evals evaluations */
     bv_step_item *d_items = nullptr;
     std::vector<int64_t> step_off;          /* [chunk][pmax + 1] item offsets      */
@@ -194,15 +198,20 @@ static int bv_layout(bv_ctx *c, int k) {
             std::vector<std::vector<bv_step_item>> q(NX);
             for (size_t i = (size_t)ch; i < all.size(); i += (size_t)NC) {
                 const int T = all[i].mp / BV_TILE, P = all[i].mp / BV_NB;
+                /* stepped: launch j covers rows 2j.. of blocks with more than
+                   j panels; lookahead (stepped == 2): launch index j is panel
+                   step j - 1, covering rows 2j.. of blocks with more than j
+                   panels (index 0: the first diagonal block alone) */
                 if (j >= P) continue;
                 size_t x = 0;
                 for (int t = 1; t < NX; ++t)
                     if (q[t].size() < q[x].size()) x = t;
-                for (int r = 2 * j; r < T; r += BV_STEP_ROWS) {
+                const int rend = (c->stepped == 2 && j == 0) ? 2 : T;
+                for (int r = 2 * j; r < rend; r += BV_STEP_ROWS) {
                     bv_step_item it;
                     it.desc = (int32_t)i;
                     it.row0 = r;
-                    it.nrows = std::min(BV_STEP_ROWS, T - r);
+                    it.nrows = std::min(BV_STEP_ROWS, rend - r);
                     it.pad_ = 0;
                     q[x].push_back(it);
                 }
@@ -309,7 +318,8 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     /* factorization kernel: BV_FACTOR_KERNEL=1w selects the one-wavefront one */
     if (const char *v = getenv("BV_FACTOR_KERNEL"))
         c->variant = (strcmp(v, "1w") == 0) ? 1 : (strcmp(v, "2w") == 0) ? 2 : 0;
-    if (const char *v = getenv("BV_PATH")) c->stepped = (strcmp(v, "stepped") == 0) ? 1 : 0;
+    if (const char *v = getenv("BV_PATH"))
+        c->stepped = (strcmp(v, "stepped") == 0) ? 1 : (strcmp(v, "stepped2") == 0) ? 2 : 0;
     if (const char *v = getenv("BV_CHUNKS")) c->nchunks = std::max(1, std::min(64, atoi(v)));
     if (const char *v = getenv("BV_CHUNK_STREAMS")) c->nstreams = std::max(1, std::min(4, atoi(v)));
     if (getenv("BV_FACTOR_KERNEL") || getenv("BV_PATH") || getenv("BV_CHUNKS")) c->autotune = 0;
@@ -424,9 +434,14 @@ static int bv_enqueue(bv_ctx *c, unsigned nblk, int kinds_used) {
                     const int64_t *off = c->step_off.data() + (size_t)ch * (c->pmax + 1);
                     const int64_t o = off[j], cnt = off[j + 1] - o;
                     if (cnt <= 0) continue;
-                    BV_HIP(bv_launch_step(kind, (unsigned)cnt, j, st, c->d_items + o, c->d_desc,
-                                          c->d_slots, c->d_ws, c->d_part_logdet, c->d_part_dot,
-                                          c->d_part_info, c->pmax));
+                    if (c->stepped == 2)
+                        BV_HIP(bv_launch_step2(kind, (unsigned)cnt, j - 1, st, c->d_items + o,
+                                               c->d_desc, c->d_slots, c->d_ws, c->d_part_logdet,
+                                               c->d_part_dot, c->d_part_info, c->pmax));
+                    else
+                        BV_HIP(bv_launch_step(kind, (unsigned)cnt, j, st, c->d_items + o,
+                                              c->d_desc, c->d_slots, c->d_ws, c->d_part_logdet,
+                                              c->d_part_dot, c->d_part_info, c->pmax));
                 }
             }
             if (side) {
@@ -569,16 +584,17 @@ static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, doubl
 
 /* Factorization configurations the autotuner chooses from: the fused kernel
    with four, one or two wavefronts per block, and the stepped
-   factorization, whole or in two or three chunks over as many streams.  Which is fastest
+   factorization, whole or in two or three chunks over as many streams, and
+   with lookahead in two or three chunks.  Which is fastest
    depends on the block sizes and the covariance kind. */
 struct bv_cfg {
     int variant, stepped, nchunks, nstreams;
 };
 static const bv_cfg bv_cfgs[] = {{0, 0, 1, 1}, {1, 0, 1, 1}, {2, 0, 1, 1}, {0, 1, 1, 1},
-                                 {0, 1, 2, 2}, {0, 1, 3, 3}};
+                                 {0, 1, 2, 2}, {0, 1, 3, 3}, {0, 2, 2, 2}, {0, 2, 3, 3}};
 
 static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
-    const bool relayout = g.nchunks != c->nchunks;
+    const bool relayout = g.nchunks != c->nchunks || g.stepped != c->stepped;
     c->variant = g.variant;
     c->stepped = g.stepped;
     c->nchunks = g.nchunks;

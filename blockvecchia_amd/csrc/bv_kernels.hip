@@ -899,6 +899,229 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
     }
 }
 
+/* ======================================================================
+ * Stepped factorization with lookahead: launches j = -1, 0, ..., P-2.
+ *
+ * Launch j (j >= 0): a work item is one wavefront on up to BV_STEP_ROWS row
+ * tiles R >= 2j+2 of one block.  It
+ *   (a) forms C(R, j) = A(R, j) - sum_{k<2j} L(R, k) L(2j..2j+1, k)^T (A
+ *       generated in registers or read for LOADA; every L from earlier
+ *       launches),
+ *   (b) solves L(R, j) = C W_j^T with W_j, the inverse of panel j's diagonal
+ *       factor, written by launch j-1, stores L(R, j) and adds row m's squares
+ *       to the (block, panel j) partial;
+ *   (c) if it holds rows 2j+2 and 2j+3 (panel j+1's diagonal block), forms
+ *       that block's C from its own rows (older columns from the workspace,
+ *       columns 2j, 2j+1 from (b) in registers), factors it in LDS, stores
+ *       W_{j+1} over its three tiles and the panel's log-determinant, row-m
+ *       squares and first failure into the (block, panel j+1) partials.
+ * Launch -1 is (c) alone for panel 0.  What an item reads was written by an
+ * earlier launch and what it writes no other item of its launch touches, so
+ * items never wait for each other.  Compared with bv_step_kernel, C never
+ * goes through the workspace and no item recomputes the panel rows.
+ * ====================================================================== */
+template <int KIND>
+__global__ void __launch_bounds__(64, BV_STEP_WAVES_PER_SIMD)
+bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
+                const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
+                double *ws_base, double *part_logdet, double *part_dot, int32_t *part_info,
+                int pmax) {
+    constexpr int G = BV_STEP_ROWS;
+    static_assert(G >= 2, "the lookahead item holds two row tiles");
+    __shared__ double sW[3 * 256]; /* diagonal C tiles in, W out */
+    __shared__ double sT[16 * 17];
+    __shared__ double sCol[16], sRow[16], sPiv[BV_NB];
+    __shared__ double s_logdet, s_dotd;
+    __shared__ int s_info;
+    __shared__ double qx[G * BV_TILE + BV_NB], qy[G * BV_TILE + BV_NB], qz[G * BV_TILE + BV_NB],
+        qo[G * BV_TILE + BV_NB];
+    __shared__ bv_kparams s_kp;
+
+    if ((int)blockIdx.x >= nitems) return;
+    const bv_step_item it = items[blockIdx.x];
+    if (it.desc < 0) return;
+    const bv_block_desc d = desc[it.desc];
+    const bv_eval_slot &E = slots[d.eval];
+    const int lane = threadIdx.x;
+    const int m = d.m, cs0 = d.cs0;
+    const int T = d.mp / BV_TILE, P = d.mp / BV_NB;
+    const int row0 = it.row0, nr = it.nrows;
+    const int jn = j + 1;                         /* the panel this launch factors */
+    const int rn0 = 2 * jn;                       /* its diagonal block's first row tile */
+    const bool look = (row0 == rn0) && (jn < P) && nr >= 2;
+    double *ws = ws_base + d.ws_off;
+    const int lr = lane & 15, lq = lane >> 4;
+
+    if (KIND != BV_KIND_LOADA) {
+        /* the item's row points, then panel j's column points */
+        if (lane < G * BV_TILE) {
+            double xv, yv, zv, ov;
+            bv_point(d, E, row0 * BV_TILE + lane, xv, yv, zv, ov);
+            qx[lane] = xv; qy[lane] = yv; qz[lane] = zv; qo[lane] = ov;
+        }
+        if (j >= 0 && lane < BV_NB) {
+            double xv, yv, zv, ov;
+            bv_point(d, E, 2 * j * BV_TILE + lane, xv, yv, zv, ov);
+            const int t = G * BV_TILE + lane;
+            qx[t] = xv; qy[t] = yv; qz[t] = zv; qo[t] = ov;
+        }
+        if (lane == 0) s_kp = E.kp;
+        bv_wave_sync();
+    }
+    const bv_kparams &kp = s_kp;
+    const bool planar = (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 || KIND == BV_KIND_NU25) &&
+                        kp.metric == 0 && !kp.use_z;
+    const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
+
+    bv_d4 xl[G][2]; /* L(R, panel j) */
+    if (j >= 0) {
+        const int rt0 = 2 * j;
+        bv_d4 acc[G][2];
+        /* (a) acc = -A(R, j), then the products of the older columns */
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            acc[s][0] = zero;
+            acc[s][1] = zero;
+            if (s >= nr) continue;
+            const int rt = row0 + s;
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                if (KIND == BV_KIND_LOADA) {
+                    acc[s][ct] = -*(const bv_d4 *)(ws + bv_tile_off(rt, rt0 + ct, T) + 4 * lane);
+                } else {
+                    const int p = rt * BV_TILE + lr;
+                    const int ip = s * BV_TILE + lr;
+                    const int q0 = (rt0 + ct) * BV_TILE;
+                    bv_d4 v;
+                    if (planar && rt * BV_TILE + 15 < m && q0 + 15 < m) {
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
+                            v[g] = bv_matern_closed<KIND>(
+                                bv_dist_euclid(qx[ip], qy[ip], 0.0, qx[iq], qy[iq], 0.0, 0), kp);
+                        }
+                    } else {
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
+                            v[g] = bv_aug_entry<KIND>(p, q0 + 4 * g + lq, m, qx[ip], qy[ip],
+                                                      qz[ip], qx[iq], qy[iq], qz[iq], qo[iq], kp);
+                        }
+                    }
+                    acc[s][ct] = -v;
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (j >= 1) {
+            switch (nr) {
+            case 1: bv_step_gemm<1, G>(ws, T, j, rt0, row0, lane, acc); break;
+#if BV_STEP_ROWS >= 2
+            case 2: bv_step_gemm<2, G>(ws, T, j, rt0, row0, lane, acc); break;
+#endif
+#if BV_STEP_ROWS >= 3
+            case 3: bv_step_gemm<3, G>(ws, T, j, rt0, row0, lane, acc); break;
+#endif
+#if BV_STEP_ROWS >= 4
+            case 4: bv_step_gemm<4, G>(ws, T, j, rt0, row0, lane, acc); break;
+#endif
+            default: break;
+            }
+        }
+        /* (b) L(R, j) = C W_j^T; row m's squares */
+        const bv_d4 w00 = *(const bv_d4 *)(ws + bv_tile_off(rt0, rt0, T) + 4 * lane);
+        const bv_d4 w10 = *(const bv_d4 *)(ws + bv_tile_off(rt0 + 1, rt0, T) + 4 * lane);
+        const bv_d4 w11 = *(const bv_d4 *)(ws + bv_tile_off(rt0 + 1, rt0 + 1, T) + 4 * lane);
+        double dot_lane = 0.0;
+        bool has_m = false;
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            if (s >= nr) continue;
+            const int rt = row0 + s;
+            bv_solve_pair(-acc[s][0], -acc[s][1], w00, w10, w11, xl[s][0], xl[s][1]);
+            *(bv_d4 *)(ws + bv_tile_off(rt, rt0, T) + 4 * lane) = xl[s][0];
+            *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + 1, T) + 4 * lane) = xl[s][1];
+            if (rt * BV_TILE <= m && m < (rt + 1) * BV_TILE) has_m = true;
+            if (rt * BV_TILE + lr == m) {
+                const int J0 = rt0 * BV_TILE;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int q0 = J0 + 4 * g + lq, q1 = q0 + BV_TILE;
+                    if (q0 >= cs0 && q0 < m) dot_lane += xl[s][0][g] * xl[s][0][g];
+                    if (q1 >= cs0 && q1 < m) dot_lane += xl[s][1][g] * xl[s][1][g];
+                }
+            }
+        }
+        if (has_m) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) dot_lane += __shfl_xor(dot_lane, off, 64);
+            if (lane == 0) part_dot[(int64_t)it.desc * pmax + j] = dot_lane;
+        }
+    }
+    if (!look) return;
+
+    /* (c) panel jn's diagonal block from the own rows s = 0, 1 */
+    bv_d4 acc2[2][2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            acc2[s][ct] = zero;
+            if (s == 0 && ct == 1) continue; /* upper tile */
+            const int rt = rn0 + s;
+            if (KIND == BV_KIND_LOADA) {
+                acc2[s][ct] = -*(const bv_d4 *)(ws + bv_tile_off(rt, rn0 + ct, T) + 4 * lane);
+            } else {
+                const int p = rt * BV_TILE + lr;
+                const int ip = s * BV_TILE + lr;
+                bv_d4 v;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int iq = ct * BV_TILE + 4 * g + lq; /* own rows are the columns */
+                    v[g] = bv_aug_entry<KIND>(p, (rn0 + ct) * BV_TILE + 4 * g + lq, m, qx[ip],
+                                              qy[ip], qz[ip], qx[iq], qy[iq], qz[iq], qo[iq], kp);
+                }
+                acc2[s][ct] = -v;
+            }
+        }
+    }
+    if (j >= 1) bv_step_gemm<2, 2>(ws, T, jn, 2 * j, rn0, lane, acc2);
+    if (j >= 0) {
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    acc2[s][0] = bv_mfma(xl[0][kk][g], xl[s][kk][g], acc2[s][0]);
+                    acc2[s][1] = bv_mfma(xl[1][kk][g], xl[s][kk][g], acc2[s][1]);
+                }
+            }
+        }
+    }
+    *(bv_d4 *)(sW + 0 * 256 + 4 * lane) = -acc2[0][0];
+    *(bv_d4 *)(sW + 1 * 256 + 4 * lane) = -acc2[1][0];
+    *(bv_d4 *)(sW + 2 * 256 + 4 * lane) = -acc2[1][1];
+    if (lane == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; }
+    bv_wave_sync();
+    bv_diag_factor_tiles(sW, sW + 256, sW + 512, lane, rn0 * BV_TILE, m, cs0, sCol, sRow, sT, sW,
+                         sPiv, &s_dotd, &s_info);
+    bv_panel_logdet(lane, rn0 * BV_TILE, m, cs0, sPiv, &s_logdet);
+    bv_wave_sync();
+    *(bv_d4 *)(ws + bv_tile_off(rn0, rn0, T) + 4 * lane) = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
+    *(bv_d4 *)(ws + bv_tile_off(rn0 + 1, rn0, T) + 4 * lane) =
+        *(const bv_d4 *)(sW + 1 * 256 + 4 * lane);
+    *(bv_d4 *)(ws + bv_tile_off(rn0 + 1, rn0 + 1, T) + 4 * lane) =
+        *(const bv_d4 *)(sW + 2 * 256 + 4 * lane);
+    if (lane == 0) {
+        const int64_t o = (int64_t)it.desc * pmax + jn;
+        part_logdet[o] = s_logdet;
+        part_dot[o] = s_dotd; /* row m in the diagonal block; else the solve of
+                                 launch jn overwrites it */
+        part_info[o] = s_info;
+    }
+}
+
 /* Per-block totals of the stepped factorization's partials, fixed order. */
 __global__ void __launch_bounds__(256)
 bv_step_parts_kernel(int ndesc, const bv_block_desc *__restrict__ desc,
@@ -1176,6 +1399,32 @@ hipError_t bv_launch_step(int kind, unsigned nitems, int j, hipStream_t st,
         break;
     default:
         hipLaunchKernelGGL(bv_step_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t bv_launch_step2(int kind, unsigned nitems, int j, hipStream_t st,
+                          const bv_step_item *items, const bv_block_desc *desc,
+                          const bv_eval_slot *slots, double *ws, double *part_logdet,
+                          double *part_dot, int32_t *part_info, int pmax) {
+    if (nitems == 0) return hipSuccess;
+    switch (kind) {
+    case BV_KIND_NU05:
+        hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_NU05>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    case BV_KIND_NU15:
+        hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_NU15>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    case BV_KIND_NU25:
+        hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_NU25>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    default:
+        hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(64), 0, st, items,
                            (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     }

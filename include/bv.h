@@ -110,7 +110,9 @@ double bv_last_kernel_ms(bv_ctx *ctx);
 
 /* The factorization configuration in use: cfg[0] = kernel variant (0: four
  * wavefronts per block, 1: one, 2: two), cfg[1] = 1 for the stepped factorization
- * (one launch per panel step over all blocks), cfg[2] = block chunks,
+ * (one launch per panel step over all blocks), 2 for the stepped factorization
+ * with lookahead (the next panel's diagonal block factored one launch
+ * early), 0 for the fused kernel, cfg[2] = block chunks,
  * cfg[3] = streams, cfg[4] = 1 when single evaluations replay a HIP graph
  * (cfg holds 5 entries).  No counterpart in the reference.  The first
  * evaluation of a context times the candidate configurations on its own data

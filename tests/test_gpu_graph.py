@@ -16,6 +16,7 @@ CONFIGS = {
     "fused_1w": {"BV_FACTOR_KERNEL": "1w"},
     "stepped": {"BV_PATH": "stepped"},
     "stepped_2chunks": {"BV_PATH": "stepped", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
+    "stepped2_2chunks": {"BV_PATH": "stepped2", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
 }
 
 

@@ -277,6 +277,9 @@ FACTOR_CONFIGS = {
     "fused_2w": {"BV_FACTOR_KERNEL": "2w"},
     "stepped": {"BV_PATH": "stepped", "BV_CHUNKS": "1", "BV_CHUNK_STREAMS": "1"},
     "stepped_3chunks_2streams": {"BV_PATH": "stepped", "BV_CHUNKS": "3", "BV_CHUNK_STREAMS": "2"},
+    "stepped2": {"BV_PATH": "stepped2", "BV_CHUNKS": "1", "BV_CHUNK_STREAMS": "1"},
+    "stepped2_3chunks_2streams": {"BV_PATH": "stepped2", "BV_CHUNKS": "3",
+                                  "BV_CHUNK_STREAMS": "2"},
 }
 
 
@@ -284,7 +287,8 @@ FACTOR_CONFIGS = {
 def test_factor_configs_match_oracle(c1_full, name, monkeypatch):
     """Every factorization configuration the autotuner can choose, fixed by
     environment (the fused kernel with four, one or two wavefronts per block;
-    the stepped factorization, whole or chunked over streams) against the
+    the stepped factorization with and without lookahead, whole or chunked
+    over streams) against the
     oracle: the n=2000 workload, a general-nu case and a large-block case."""
     for k, v in FACTOR_CONFIGS[name].items():
         monkeypatch.setenv(k, v)
