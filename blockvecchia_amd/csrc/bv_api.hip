@@ -921,6 +921,17 @@ extern "C" int bv_server_attach(bv_server *s, bv_ctx *dataset, int *dataset_id) 
     if (dataset->device != s->device) return BV_ERR_ARG;
     s->datasets.push_back(dataset);
     *dataset_id = (int)s->datasets.size() - 1;
+    /* size the idle lanes for a full batch now: growing a lane later frees
+       device memory, which waits for every lane's work */
+    int64_t max_b = 0;
+    for (const bv_ctx *c : s->datasets) max_b = std::max<int64_t>(max_b, c->B);
+    const int64_t ws = std::max<int64_t>((int64_t)(s->ws_budget / 8.0), dataset->ws_per_eval);
+    BV_HIP(hipSetDevice(s->device));
+    for (bv_lane &L : s->lanes) {
+        if (L.busy) continue;
+        const int rc = bv_lane_reserve(L, ws, max_b * s->max_evals, s->max_evals);
+        if (rc != BV_OK) return rc;
+    }
     return BV_OK;
 }
 

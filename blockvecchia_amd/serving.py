@@ -8,6 +8,10 @@ sequence one evaluation at a time with bv_llh.  Reported: evaluations/s of
 both, the served p50/p99 latency from submit to result, the plain per-call
 p50/p99, and the largest relative difference between a served result and the
 same evaluation alone.
+
+The open-loop scenario (benchmark_poisson) instead sends requests at Poisson
+arrival times at a fixed offered rate, to the server and, in arrival order,
+to the plain form; latency counts from arrival, so it includes queueing.
 """
 import time
 
@@ -121,6 +125,93 @@ def run_served(engines, plan, lanes=4, ws_bytes=2 << 30, max_evals=64):
     return out, wall, np.array(lat)
 
 
+def poisson_requests(truths, count, rate, seed=2):
+    """count requests (dataset index, theta near its truth) with Poisson
+    arrival times at `rate` per second."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    reqs = []
+    for i in range(count):
+        d = int(rng.integers(len(truths)))
+        th = truths[d].copy()
+        th[0] *= 1.0 + 0.05 * rng.standard_normal()
+        th[1] *= 1.0 + 0.05 * rng.standard_normal()
+        reqs.append((float(t[i]), d, th))
+    return reqs
+
+
+def run_open_plain(engines, reqs):
+    """Requests evaluated one at a time in arrival order, each as soon as it
+    has arrived and the previous one is done; latency from arrival."""
+    lat = np.empty(len(reqs))
+    out = {}
+    t0 = time.perf_counter()
+    for i, (ta, d, th) in enumerate(reqs):
+        while time.perf_counter() - t0 < ta:
+            pass
+        out[i] = engines[d].llh(th)
+        lat[i] = time.perf_counter() - t0 - ta
+    return out, time.perf_counter() - t0, lat
+
+
+def run_open_served(engines, reqs, lanes=4, ws_bytes=2 << 30, max_evals=64):
+    """Requests submitted to the server at their arrival times, results
+    collected as lanes finish; latency from arrival."""
+    srv = bv.Server(lanes=lanes, ws_bytes_per_lane=ws_bytes, max_evals_per_batch=max_evals)
+    ids = [srv.attach(e) for e in engines]
+    lat = np.empty(len(reqs))
+    out = {}
+    nxt = 0
+    done = 0
+    t0 = time.perf_counter()
+    while done < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and reqs[nxt][0] <= now:
+            srv.submit(ids[reqs[nxt][1]], reqs[nxt][2], nxt)
+            nxt += 1
+        res = srv.poll(wait=False)
+        if not res and nxt == len(reqs):
+            res = srv.poll(wait=True)
+        now = time.perf_counter() - t0
+        for tag, v, st in res:
+            out[tag] = v
+            lat[tag] = now - reqs[tag][0]
+            done += 1
+    wall = time.perf_counter() - t0
+    srv.close()
+    return out, wall, lat
+
+
+def benchmark_poisson(count=4096, load=0.6, lanes=4, specs=SMALL_DATASETS, seed=2):
+    """Open-loop arrivals: `count` requests over the datasets at a Poisson
+    rate of `load` times the server's closed-loop capacity, served and
+    plain on the same arrival stream.  Past its own capacity the plain form
+    queues, so its latency grows with the run; the served form keeps up."""
+    engines, truths = build_datasets(specs)
+    plan = make_requests(truths, 64, 16)
+    run_served(engines, plan, lanes)
+    _, t_cap, _ = run_served(engines, plan, lanes)
+    capacity = sum(len(ths) for _, ths in plan) / t_cap
+    rate = load * capacity
+    reqs = poisson_requests(truths, count, rate, seed)
+    run_open_served(engines, reqs, lanes)                # warm-up: lane buffers at full size
+    served, t_s, lat_s = run_open_served(engines, reqs, lanes)
+    plain, t_p, lat_p = run_open_plain(engines, reqs)
+    worst = max(abs(served[k] - plain[k]) / abs(plain[k]) for k in plain)
+    for e in engines:
+        e.close()
+    return {
+        "requests": count, "offered_rate_per_s": rate, "server_capacity_per_s": capacity,
+        "evals_per_s": count / t_s, "plain_evals_per_s": count / t_p,
+        "speedup_vs_plain": t_p / t_s,
+        "latency_ms": {"p50": float(np.percentile(lat_s, 50) * 1e3),
+                       "p99": float(np.percentile(lat_s, 99) * 1e3)},
+        "plain_latency_ms": {"p50": float(np.percentile(lat_p, 50) * 1e3),
+                             "p99": float(np.percentile(lat_p, 99) * 1e3)},
+        "max_rel_diff_vs_alone": worst,
+    }
+
+
 def benchmark(sessions=32, per_session=16, lanes=4, ws_bytes=2 << 30, max_evals=64,
               specs=DEFAULT_DATASETS):
     engines, truths = build_datasets(specs)
@@ -155,6 +246,7 @@ def scenarios():
     return {
         "small_datasets": benchmark(sessions=64, per_session=16, specs=SMALL_DATASETS),
         "mixed_sizes": benchmark(sessions=32, per_session=16, specs=DEFAULT_DATASETS),
+        "poisson_small_datasets": benchmark_poisson(),
     }
 
 

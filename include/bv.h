@@ -157,7 +157,9 @@ int bv_server_create(int device, int nlanes, double ws_bytes_per_lane,
                      int max_evals_per_batch, bv_server **out);
 
 /* Register a dataset (the context stays owned by the caller and must outlive
- * the server's use of it). */
+ * the server's use of it).  Idle lanes are sized here for a full batch
+ * (ws_bytes_per_lane of workspace, max_evals_per_batch evaluations of the
+ * largest attached dataset), so that serving does not grow them later. */
 int bv_server_attach(bv_server *s, bv_ctx *dataset, int *dataset_id);
 
 /* Queue one evaluation; tag is returned with its result. */
