@@ -654,6 +654,44 @@ __device__ __forceinline__ void bv_step_gemm(const double *__restrict__ ws, int 
     }
 }
 
+/* bv_step_gemm for two row tiles rbase, rbase + 1 that also accumulate
+   their own lower 2x2 tile block (the SYRK of the next panel's diagonal
+   block: d00 += r0 r0^T, d10 += r1 r0^T, d11 += r1 r1^T) from the same
+   loaded tiles. */
+template <int N>
+__device__ __forceinline__ void bv_step_gemm_syrk(const double *__restrict__ ws, int T, int j,
+                                                  int kend, int rbase, int lane,
+                                                  bv_d4 (&acc)[N][2], bv_d4 &d00, bv_d4 &d10,
+                                                  bv_d4 &d11) {
+    const int rt0 = 2 * j;
+    if (kend <= 0) return;
+    const int lo = 4 * lane;
+    const double *colp = ws;
+    bv_d4 ja0, ja1, ra[2], jb0, jb1, rb[2];
+    bv_gemm_load<2>(colp, rt0, rbase, 1, lo, ja0, ja1, ra);
+    for (int kt = 0; kt < kend; kt += 2) {
+        const double *colb = colp + ((int64_t)(T - kt - 1) << 8);
+        bv_gemm_load<2>(colb, rt0, rbase, 1, lo, jb0, jb1, rb);
+        bv_gemm_step<2, N>(ja0, ja1, ra, acc);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            d00 = bv_mfma(ra[0][g], ra[0][g], d00);
+            d10 = bv_mfma(ra[0][g], ra[1][g], d10);
+            d11 = bv_mfma(ra[1][g], ra[1][g], d11);
+        }
+        const double *cola = (kt + 2 < kend) ? colb + ((int64_t)(T - kt - 2) << 8) : colb;
+        bv_gemm_load<2>(cola, rt0, rbase, 1, lo, ja0, ja1, ra);
+        bv_gemm_step<2, N>(jb0, jb1, rb, acc);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            d00 = bv_mfma(rb[0][g], rb[0][g], d00);
+            d10 = bv_mfma(rb[0][g], rb[1][g], d10);
+            d11 = bv_mfma(rb[1][g], rb[1][g], d11);
+        }
+        colp = cola;
+    }
+}
+
 /* X = C W^T for one row tile (two column tiles), W = the three tiles of a
    diagonal block's inverse. */
 __device__ __forceinline__ void bv_solve_pair(const bv_d4 &c0, const bv_d4 &c1, const bv_d4 &w00,
@@ -974,6 +1012,10 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
     const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
 
     bv_d4 xl[G][2]; /* L(R, panel j) */
+    /* the lookahead item's products for panel jn's diagonal block over the
+       columns before 2j, accumulated in (a)'s pass over its own rows */
+    bv_d4 d00 = zero, d10 = zero, d11 = zero;
+    const bool fused = look && nr == 2 && j >= 1;
     if (j >= 0) {
         const int rt0 = 2 * j;
         bv_d4 acc[G][2];
@@ -1013,7 +1055,9 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-        if (j >= 1) {
+        if (fused) {
+            bv_step_gemm_syrk<G>(ws, T, j, rt0, row0, lane, acc, d00, d10, d11);
+        } else if (j >= 1) {
             switch (nr) {
             case 1: bv_step_gemm<1, G>(ws, T, j, rt0, row0, lane, acc); break;
 #if BV_STEP_ROWS >= 2
@@ -1074,18 +1118,34 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
             } else {
                 const int p = rt * BV_TILE + lr;
                 const int ip = s * BV_TILE + lr;
+                const int q0 = (rn0 + ct) * BV_TILE;
                 bv_d4 v;
+                if (planar && rt * BV_TILE + 15 < m && q0 + 15 < m) {
 #pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const int iq = ct * BV_TILE + 4 * g + lq; /* own rows are the columns */
-                    v[g] = bv_aug_entry<KIND>(p, (rn0 + ct) * BV_TILE + 4 * g + lq, m, qx[ip],
-                                              qy[ip], qz[ip], qx[iq], qy[iq], qz[iq], qo[iq], kp);
+                    for (int g = 0; g < 4; ++g) {
+                        const int iq = ct * BV_TILE + 4 * g + lq; /* own rows are the columns */
+                        v[g] = bv_matern_closed<KIND>(
+                            bv_dist_euclid(qx[ip], qy[ip], 0.0, qx[iq], qy[iq], 0.0, 0), kp);
+                    }
+                } else {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int iq = ct * BV_TILE + 4 * g + lq;
+                        v[g] = bv_aug_entry<KIND>(p, q0 + 4 * g + lq, m, qx[ip], qy[ip], qz[ip],
+                                                  qx[iq], qy[iq], qz[iq], qo[iq], kp);
+                    }
                 }
                 acc2[s][ct] = -v;
             }
         }
     }
-    if (j >= 1) bv_step_gemm<2, 2>(ws, T, jn, 2 * j, rn0, lane, acc2);
+    if (fused) {
+        acc2[0][0] += d00;
+        acc2[1][0] += d10;
+        acc2[1][1] += d11;
+    } else if (j >= 1) {
+        bv_step_gemm<2, 2>(ws, T, jn, 2 * j, rn0, lane, acc2);
+    }
     if (j >= 0) {
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
