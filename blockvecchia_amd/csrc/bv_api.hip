@@ -38,6 +38,10 @@ hipError_t bv_launch_step2(int kind, unsigned nitems, int j, hipStream_t st,
                            const bv_step_item *items, const bv_block_desc *desc,
                            const bv_eval_slot *slots, double *ws, double *part_logdet,
                            double *part_dot, int32_t *part_info, int pmax);
+hipError_t bv_launch_pair(int kind, unsigned nitems, int s, hipStream_t st,
+                          const bv_step_item *items, const bv_block_desc *desc,
+                          const bv_eval_slot *slots, double *ws, double *part_logdet,
+                          double *part_dot, int32_t *part_info, int pmax);
 hipError_t bv_launch_step_parts(int ndesc, hipStream_t st, const bv_block_desc *desc,
                                 const bv_eval_slot *slots, const double *part_logdet,
                                 const double *part_dot, const int32_t *part_info, int pmax);
@@ -92,11 +96,13 @@ struct bv_ctx {
     double last_ms = 0.0;
     int iterations = 0;
     int variant = 0; /* factorization kernel: wavefronts per block 0: four, 1: one, 2: two */
-    int stepped = 0; /* one launch per panel step over all blocks: 1, or 2 with lookahead */
+    int stepped = 0; /* one launch per panel step over all blocks: 1, or 2 with lookahead;
+                        3: one launch per pair of panels with lookahead */
     /* stepped factorization: work items of every step for desc_evals evaluations */
     bv_step_item *d_items = nullptr;
     std::vector<int64_t> step_off;          /* [chunk][pmax + 1] item offsets      */
     int nchunks = 1, nstreams = 1;          /* block chunks, and streams they cycle over */
+    int look_first = 1;                     /* lookahead items dispatched first */
     int autotune = 1, tuned = 0;            /* choose the configuration on first use */
     hipStream_t cstream[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t cev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -195,37 +201,48 @@ static int bv_layout(bv_ctx *c, int k) {
     for (int ch = 0; ch < NC; ++ch) {
         for (int j = 0; j < c->pmax; ++j) {
             c->step_off[(size_t)ch * (c->pmax + 1) + j] = (int64_t)items.size();
-            std::vector<std::vector<bv_step_item>> q(NX);
+            /* qa: the items that factor a diagonal block ahead (lookahead
+               forms), dispatched before every other item of the launch so
+               that their serial factorizations start first; qb: the rest */
+            std::vector<std::vector<bv_step_item>> qa(NX), qb(NX);
             for (size_t i = (size_t)ch; i < all.size(); i += (size_t)NC) {
                 const int T = all[i].mp / BV_TILE, P = all[i].mp / BV_NB;
                 /* stepped: launch j covers rows 2j.. of blocks with more than
                    j panels; lookahead (stepped == 2): launch index j is panel
                    step j - 1, covering rows 2j.. of blocks with more than j
-                   panels (index 0: the first diagonal block alone) */
-                if (j >= P) continue;
+                   panels (index 0: the first diagonal block alone); paired
+                   (stepped == 3): launch index j is pair step j - 1, covering
+                   rows 4j.. in workgroups of four row tiles (index 0: the
+                   first pair's diagonal block alone) */
+                const int rows = (c->stepped == 3) ? 4 : BV_STEP_ROWS;
+                const int r0 = (c->stepped == 3) ? 4 * j : 2 * j;
+                if (c->stepped == 3 ? r0 >= T : j >= P) continue;
                 size_t x = 0;
                 for (int t = 1; t < NX; ++t)
-                    if (q[t].size() < q[x].size()) x = t;
-                const int rend = (c->stepped == 2 && j == 0) ? 2 : T;
-                for (int r = 2 * j; r < rend; r += BV_STEP_ROWS) {
+                    if (qa[t].size() + qb[t].size() < qa[x].size() + qb[x].size()) x = t;
+                const int rend = (c->stepped >= 2 && j == 0) ? std::min(c->stepped == 3 ? 4 : 2, T) : T;
+                for (int r = r0; r < rend; r += rows) {
                     bv_step_item it;
                     it.desc = (int32_t)i;
                     it.row0 = r;
-                    it.nrows = std::min(BV_STEP_ROWS, rend - r);
+                    it.nrows = std::min(rows, rend - r);
                     it.pad_ = 0;
-                    q[x].push_back(it);
+                    const bool ahead = c->look_first && c->stepped >= 2 && r == r0;
+                    (ahead ? qa : qb)[x].push_back(it);
                 }
             }
-            size_t len = 0;
-            for (int t = 0; t < NX; ++t) len = std::max(len, q[t].size());
-            for (size_t w = 0; w < len * NX; ++w) {
-                const std::vector<bv_step_item> &qq = q[w % NX];
-                const size_t k2 = w / NX;
-                if (k2 < qq.size()) items.push_back(qq[k2]);
-                else {
-                    bv_step_item e;
-                    e.desc = -1; e.row0 = 0; e.nrows = 0; e.pad_ = 0;
-                    items.push_back(e);
+            for (const std::vector<std::vector<bv_step_item>> *qs : {&qa, &qb}) {
+                size_t len = 0;
+                for (int t = 0; t < NX; ++t) len = std::max(len, (*qs)[t].size());
+                for (size_t w = 0; w < len * NX; ++w) {
+                    const std::vector<bv_step_item> &qq = (*qs)[w % NX];
+                    const size_t k2 = w / NX;
+                    if (k2 < qq.size()) items.push_back(qq[k2]);
+                    else {
+                        bv_step_item e;
+                        e.desc = -1; e.row0 = 0; e.nrows = 0; e.pad_ = 0;
+                        items.push_back(e);
+                    }
                 }
             }
         }
@@ -319,12 +336,16 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     if (const char *v = getenv("BV_FACTOR_KERNEL"))
         c->variant = (strcmp(v, "1w") == 0) ? 1 : (strcmp(v, "2w") == 0) ? 2 : 0;
     if (const char *v = getenv("BV_PATH"))
-        c->stepped = (strcmp(v, "stepped") == 0) ? 1 : (strcmp(v, "stepped2") == 0) ? 2 : 0;
+        c->stepped = (strcmp(v, "stepped") == 0)    ? 1
+                     : (strcmp(v, "stepped2") == 0) ? 2
+                     : (strcmp(v, "paired") == 0)   ? 3
+                                                    : 0;
     if (const char *v = getenv("BV_CHUNKS")) c->nchunks = std::max(1, std::min(64, atoi(v)));
     if (const char *v = getenv("BV_CHUNK_STREAMS")) c->nstreams = std::max(1, std::min(4, atoi(v)));
     if (getenv("BV_FACTOR_KERNEL") || getenv("BV_PATH") || getenv("BV_CHUNKS")) c->autotune = 0;
     if (const char *v = getenv("BV_AUTOTUNE")) c->autotune = c->autotune && atoi(v) != 0;
     if (const char *v = getenv("BV_GRAPH")) c->use_graph = atoi(v) != 0;
+    if (const char *v = getenv("BV_LOOK_FIRST")) c->look_first = atoi(v) != 0;
     c->h_batch_num.assign(p->batch_num, p->batch_num + p->batch_count);
     c->h_accum.assign((size_t)c->B + 1, 0);
     for (int i = 0; i < c->B; ++i) {
@@ -434,7 +455,11 @@ static int bv_enqueue(bv_ctx *c, unsigned nblk, int kinds_used) {
                     const int64_t *off = c->step_off.data() + (size_t)ch * (c->pmax + 1);
                     const int64_t o = off[j], cnt = off[j + 1] - o;
                     if (cnt <= 0) continue;
-                    if (c->stepped == 2)
+                    if (c->stepped == 3)
+                        BV_HIP(bv_launch_pair(kind, (unsigned)cnt, j - 1, st, c->d_items + o,
+                                              c->d_desc, c->d_slots, c->d_ws, c->d_part_logdet,
+                                              c->d_part_dot, c->d_part_info, c->pmax));
+                    else if (c->stepped == 2)
                         BV_HIP(bv_launch_step2(kind, (unsigned)cnt, j - 1, st, c->d_items + o,
                                                c->d_desc, c->d_slots, c->d_ws, c->d_part_logdet,
                                                c->d_part_dot, c->d_part_info, c->pmax));
@@ -584,14 +609,16 @@ static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, doubl
 
 /* Factorization configurations the autotuner chooses from: the fused kernel
    with four, one or two wavefronts per block, and the stepped
-   factorization, whole or in two or three chunks over as many streams, and
-   with lookahead in two or three chunks.  Which is fastest
-   depends on the block sizes and the covariance kind. */
+   factorization, whole or in two or three chunks over as many streams, with
+   lookahead in two or three chunks, and by pairs of panels whole or in two
+   or three chunks.  Which is fastest depends on the block sizes and the
+   covariance kind. */
 struct bv_cfg {
     int variant, stepped, nchunks, nstreams;
 };
 static const bv_cfg bv_cfgs[] = {{0, 0, 1, 1}, {1, 0, 1, 1}, {2, 0, 1, 1}, {0, 1, 1, 1},
-                                 {0, 1, 2, 2}, {0, 1, 3, 3}, {0, 2, 2, 2}, {0, 2, 3, 3}};
+                                 {0, 1, 2, 2}, {0, 1, 3, 3}, {0, 2, 2, 2}, {0, 2, 3, 3},
+                                 {0, 3, 1, 1}, {0, 3, 2, 2}, {0, 3, 3, 3}};
 
 static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
     const bool relayout = g.nchunks != c->nchunks || g.stepped != c->stepped;

@@ -1182,6 +1182,409 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
     }
 }
 
+/* ======================================================================
+ * Paired-panel factorization: one launch per pair of 32-column panels
+ * (2s, 2s+1), i.e. tile columns c0 = 4s .. 4s+3, with a lookahead of one
+ * pair.  A workgroup is two wavefronts on four consecutive row tiles of one
+ * block (wave w: rows row0 + 2w, row0 + 2w + 1), all rows >= 4s + 4.  At
+ * launch s, with W = L_pp^-1 of panels 2s and 2s+1 and L(panel 2s+1 rows,
+ * panel 2s) from launch s-1, each wave
+ *   (a) forms C(R, pair) = A(R, pair) - sum_{k < c0} L(R, k) L(pair rows, k)^T
+ *       (four output column tiles per load of its own row tiles: twice the
+ *       products per byte of the one-panel steps),
+ *   (b) solves panel 2s, L(R, 2s) = C(R, 2s) W_2s^T, subtracts
+ *       L(R, 2s) L(panel 2s+1 rows, 2s)^T from C(R, 2s+1) and solves panel
+ *       2s+1; stores L(R, pair) and adds row m's squares per panel.
+ * The workgroup on rows 4s+4 .. 4s+7 (the next pair's rows Q = A u B) also
+ * factors the next pair's 64x64 diagonal block D = A(Q,Q) - L(Q, <c0+4)
+ * L(Q, <c0+4)^T: wave A's 2x2 tile block is accumulated with its pass (a),
+ * wave B's own block likewise and its cross block D_BA by a second pass over
+ * the columns < c0 while wave A factors D_AA (W_A); then wave B forms
+ * L_BA = D_BA W_A^T, D_BB - L_BA L_BA^T and factors it (W_B).  W_A, L_BA and
+ * W_B go to the workspace with the panels' log-determinants, row-m squares
+ * and first failures in the (block, panel) partials.  Launch -1 is that
+ * factorization alone for pair 0.  Nothing a workgroup reads is written by
+ * another workgroup of the same launch: workgroups never wait for each other.
+ * ====================================================================== */
+#ifndef BV_PAIR_WAVES_PER_SIMD
+#define BV_PAIR_WAVES_PER_SIMD 2
+#endif
+
+/* The pair's four row tiles and two own row tiles of one tile column. */
+__device__ __forceinline__ void bv_pair_load(const double *colp, int c0, int rbase, int lo,
+                                             bv_d4 (&p)[4], bv_d4 (&r)[2]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) p[c] = *(const bv_d4 *)(colp + ((int64_t)(c0 + c) << 8) + lo);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) r[i] = *(const bv_d4 *)(colp + ((int64_t)(rbase + i) << 8) + lo);
+}
+
+template <bool SYRK>
+__device__ __forceinline__ void bv_pair_step(const bv_d4 (&p)[4], const bv_d4 (&r)[2],
+                                             bv_d4 (&acc)[2][4], bv_d4 &d00, bv_d4 &d10,
+                                             bv_d4 &d11) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[i][c] = bv_mfma(p[c][g], r[i][g], acc[i][c]);
+        }
+        if (SYRK) {
+            d00 = bv_mfma(r[0][g], r[0][g], d00);
+            d10 = bv_mfma(r[0][g], r[1][g], d10);
+            d11 = bv_mfma(r[1][g], r[1][g], d11);
+        }
+    }
+}
+
+/* acc[i][c] += L(rbase + i, k) L(c0 + c, k)^T over the tile columns k < kend
+   (kend even), and with SYRK the own lower 2x2 tile block from the same
+   loaded tiles.  Two operand buffers alternate: the loads of column k+1 are in
+   flight while column k's products issue. */
+template <bool SYRK>
+__device__ __forceinline__ void bv_pair_gemm(const double *__restrict__ ws, int T, int c0, int kend,
+                                             int rbase, int lane, bv_d4 (&acc)[2][4], bv_d4 &d00,
+                                             bv_d4 &d10, bv_d4 &d11) {
+    if (kend <= 0) return;
+    const int lo = 4 * lane;
+    const double *colp = ws;
+    bv_d4 pa[4], ra[2], pb[4], rb[2];
+    bv_pair_load(colp, c0, rbase, lo, pa, ra);
+    for (int kt = 0; kt < kend; kt += 2) {
+        const double *colb = colp + ((int64_t)(T - kt - 1) << 8);
+        bv_pair_load(colb, c0, rbase, lo, pb, rb);
+        bv_pair_step<SYRK>(pa, ra, acc, d00, d10, d11);
+        const double *cola = (kt + 2 < kend) ? colb + ((int64_t)(T - kt - 2) << 8) : colb;
+        bv_pair_load(cola, c0, rbase, lo, pa, ra);
+        bv_pair_step<SYRK>(pb, rb, acc, d00, d10, d11);
+        colp = cola;
+    }
+}
+
+/* Tile (rt, ct) of the augmented matrix: generated from the points staged in
+   LDS (row points from index ip0, column points from iq0) or, for LOADA, read
+   from the workspace. */
+template <int KIND>
+__device__ __forceinline__ bv_d4 bv_pair_tile(const double *__restrict__ ws, int T, int rt, int ct,
+                                              int m, int ip0, int iq0, int lane,
+                                              const double *qx, const double *qy,
+                                              const double *qz, const double *qo,
+                                              const bv_kparams &kp, bool planar) {
+    if (KIND == BV_KIND_LOADA) return *(const bv_d4 *)(ws + bv_tile_off(rt, ct, T) + 4 * lane);
+    const int lr = lane & 15, lq = lane >> 4;
+    const int p = rt * BV_TILE + lr, ip = ip0 + lr, q0 = ct * BV_TILE;
+    bv_d4 v;
+    if (planar && rt * BV_TILE + 15 < m && q0 + 15 < m) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int iq = iq0 + 4 * g + lq;
+            v[g] = bv_matern_closed<KIND>(bv_dist_euclid(qx[ip], qy[ip], 0.0, qx[iq], qy[iq], 0.0, 0),
+                                          kp);
+        }
+    } else {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int iq = iq0 + 4 * g + lq;
+            v[g] = bv_aug_entry<KIND>(p, q0 + 4 * g + lq, m, qx[ip], qy[ip], qz[ip], qx[iq], qy[iq],
+                                      qz[iq], qo[iq], kp);
+        }
+    }
+    return v;
+}
+
+/* Squares of row m's entries in columns [J0, J0 + 16 NC) that are >= cs0 and
+   < m, for the lanes holding row m of tile rt (else 0). */
+template <int NC>
+__device__ __forceinline__ double bv_row_m_sq(const bv_d4 (&x)[NC], int rt, int J0, int m, int cs0,
+                                              int lane) {
+    const int lr = lane & 15, lq = lane >> 4;
+    double t = 0.0;
+    if (rt * BV_TILE + lr == m) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int q = J0 + c * BV_TILE + 4 * g + lq;
+                if (q >= cs0 && q < m) t += x[c][g] * x[c][g];
+            }
+        }
+    }
+    return t;
+}
+
+__device__ __forceinline__ double bv_wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(128, BV_PAIR_WAVES_PER_SIMD)
+bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems, int s,
+               const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
+               double *ws_base, double *part_logdet, double *part_dot, int32_t *part_info,
+               int pmax) {
+    __shared__ double qx[128], qy[128], qz[128], qo[128]; /* row points, then the pair's columns */
+    __shared__ double sX[8 * 256];                        /* wave A's L(Q_A, pair) for wave B */
+    __shared__ double sW[2][3 * 256];                     /* each wave's diagonal block, W out */
+    __shared__ double sT[2][16 * 17];
+    __shared__ double sCol[2][16], sRow[2][16], sPiv[2][BV_NB];
+    __shared__ double s_logdet[2], s_dotd[2];
+    __shared__ int s_info[2];
+    __shared__ bv_kparams s_kp;
+
+    if ((int)blockIdx.x >= nitems) return;
+    const bv_step_item it = items[blockIdx.x];
+    if (it.desc < 0) return;
+    const bv_block_desc d = desc[it.desc];
+    const bv_eval_slot &E = slots[d.eval];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int m = d.m, cs0 = d.cs0;
+    const int T = d.mp / BV_TILE;
+    const int row0 = it.row0, nr = it.nrows;
+    const int c0 = 4 * s;
+    const bool look = row0 == 4 * s + 4; /* this workgroup holds the next pair's rows */
+    const int myr = row0 + 2 * w;
+    const bool mine = 2 * w < nr;
+    double *ws = ws_base + d.ws_off;
+
+    if (KIND != BV_KIND_LOADA) {
+        if (tid < 64 || s >= 0) {
+            const int p = (tid < 64) ? row0 * BV_TILE + tid : c0 * BV_TILE + (tid - 64);
+            double xv, yv, zv, ov;
+            bv_point(d, E, p, xv, yv, zv, ov);
+            qx[tid] = xv; qy[tid] = yv; qz[tid] = zv; qo[tid] = ov;
+        }
+        if (tid == 0) s_kp = E.kp;
+    }
+    __syncthreads();
+    const bv_kparams &kp = s_kp;
+    const bool planar = (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 || KIND == BV_KIND_NU25) &&
+                        kp.metric == 0 && !kp.use_z;
+    const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
+
+    bv_d4 x[2][4];                          /* L(own rows, pair) */
+    bv_d4 d00 = zero, d10 = zero, d11 = zero; /* own rows' SYRK over k < c0 */
+    if (s >= 0 && mine) {
+        /* -A(R, pair), written out: the generation is too large for the
+           unroller, and a rolled loop would index the accumulators
+           dynamically (scratch memory) */
+        auto gen = [&](int i, int c) {
+            return -bv_pair_tile<KIND>(ws, T, myr + i, c0 + c, m, (2 * w + i) * BV_TILE,
+                                       64 + c * BV_TILE, lane, qx, qy, qz, qo, kp, planar);
+        };
+        bv_d4 acc[2][4] = {{gen(0, 0), gen(0, 1), gen(0, 2), gen(0, 3)},
+                           {gen(1, 0), gen(1, 1), gen(1, 2), gen(1, 3)}};
+        if (look) bv_pair_gemm<true>(ws, T, c0, c0, myr, lane, acc, d00, d10, d11);
+        else bv_pair_gemm<false>(ws, T, c0, c0, myr, lane, acc, d00, d10, d11);
+        const int lo = 4 * lane;
+        {
+            const bv_d4 w00 = *(const bv_d4 *)(ws + bv_tile_off(c0, c0, T) + lo);
+            const bv_d4 w10 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 1, c0, T) + lo);
+            const bv_d4 w11 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 1, c0 + 1, T) + lo);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                bv_solve_pair(-acc[i][0], -acc[i][1], w00, w10, w11, x[i][0], x[i][1]);
+        }
+        {
+            /* C(R, 2s+1) -= L(R, 2s) L(panel 2s+1 rows, 2s)^T */
+            const bv_d4 l00 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 2, c0, T) + lo);
+            const bv_d4 l01 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 2, c0 + 1, T) + lo);
+            const bv_d4 l10 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 3, c0, T) + lo);
+            const bv_d4 l11 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 3, c0 + 1, T) + lo);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    acc[i][2] = bv_mfma(l00[g], x[i][0][g], acc[i][2]);
+                    acc[i][2] = bv_mfma(l01[g], x[i][1][g], acc[i][2]);
+                    acc[i][3] = bv_mfma(l10[g], x[i][0][g], acc[i][3]);
+                    acc[i][3] = bv_mfma(l11[g], x[i][1][g], acc[i][3]);
+                }
+            }
+        }
+        {
+            const bv_d4 w00 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 2, c0 + 2, T) + lo);
+            const bv_d4 w10 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 3, c0 + 2, T) + lo);
+            const bv_d4 w11 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 3, c0 + 3, T) + lo);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                bv_solve_pair(-acc[i][2], -acc[i][3], w00, w10, w11, x[i][2], x[i][3]);
+        }
+        bool has_m = false;
+        double dp0 = 0.0, dp1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int rt = myr + i;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                *(bv_d4 *)(ws + bv_tile_off(rt, c0 + c, T) + lo) = x[i][c];
+            if (rt * BV_TILE <= m && m < (rt + 1) * BV_TILE) has_m = true;
+            const bv_d4 xa[2] = {x[i][0], x[i][1]};
+            const bv_d4 xb[2] = {x[i][2], x[i][3]};
+            dp0 += bv_row_m_sq<2>(xa, rt, c0 * BV_TILE, m, cs0, lane);
+            dp1 += bv_row_m_sq<2>(xb, rt, (c0 + 2) * BV_TILE, m, cs0, lane);
+        }
+        if (has_m) {
+            dp0 = bv_wave_sum(dp0);
+            dp1 = bv_wave_sum(dp1);
+            if (lane == 0) {
+                part_dot[(int64_t)it.desc * pmax + 2 * s] = dp0;
+                part_dot[(int64_t)it.desc * pmax + 2 * s + 1] = dp1;
+            }
+        }
+    }
+    if (!look) return;
+
+    /* the next pair's diagonal block: Q = rows qn .. qn+3 (wave A: qn, qn+1;
+       wave B: qn+2, qn+3 when they exist); accumulators hold -D */
+    const int qn = row0;
+    const bool hasB = qn + 2 < T;
+    const int pa = qn / 2;
+    bv_d4 db0 = zero, db1 = zero, db2 = zero; /* wave B: D_BB */
+    bv_d4 e[2][2];                            /* wave B: D_BA */
+    if (w == 0) {
+        bv_d4 da0 = d00 - bv_pair_tile<KIND>(ws, T, qn, qn, m, 0, 0, lane, qx, qy, qz, qo, kp, planar);
+        bv_d4 da1 = d10 - bv_pair_tile<KIND>(ws, T, qn + 1, qn, m, 16, 0, lane, qx, qy, qz, qo, kp,
+                                             planar);
+        bv_d4 da2 = d11 - bv_pair_tile<KIND>(ws, T, qn + 1, qn + 1, m, 16, 16, lane, qx, qy, qz, qo,
+                                             kp, planar);
+        if (s >= 0) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    da0 = bv_mfma(x[0][kk][g], x[0][kk][g], da0);
+                    da1 = bv_mfma(x[0][kk][g], x[1][kk][g], da1);
+                    da2 = bv_mfma(x[1][kk][g], x[1][kk][g], da2);
+                }
+            }
+            if (hasB) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        *(bv_d4 *)(sX + (i * 4 + c) * 256 + 4 * lane) = x[i][c];
+                }
+            }
+        }
+        *(bv_d4 *)(sW[0] + 0 * 256 + 4 * lane) = -da0;
+        *(bv_d4 *)(sW[0] + 1 * 256 + 4 * lane) = -da1;
+        *(bv_d4 *)(sW[0] + 2 * 256 + 4 * lane) = -da2;
+        if (lane == 0) { s_logdet[0] = 0.0; s_dotd[0] = 0.0; s_info[0] = 0; }
+    } else if (hasB) {
+        db0 = d00 - bv_pair_tile<KIND>(ws, T, qn + 2, qn + 2, m, 32, 32, lane, qx, qy, qz, qo, kp,
+                                       planar);
+        db1 = d10 - bv_pair_tile<KIND>(ws, T, qn + 3, qn + 2, m, 48, 32, lane, qx, qy, qz, qo, kp,
+                                       planar);
+        db2 = d11 - bv_pair_tile<KIND>(ws, T, qn + 3, qn + 3, m, 48, 48, lane, qx, qy, qz, qo, kp,
+                                       planar);
+        if (s >= 0) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    db0 = bv_mfma(x[0][kk][g], x[0][kk][g], db0);
+                    db1 = bv_mfma(x[0][kk][g], x[1][kk][g], db1);
+                    db2 = bv_mfma(x[1][kk][g], x[1][kk][g], db2);
+                }
+            }
+        }
+        if (lane == 0) { s_logdet[1] = 0.0; s_dotd[1] = 0.0; s_info[1] = 0; }
+    }
+    __syncthreads(); /* sX, sW[0] */
+    if (w == 0) {
+        bv_diag_factor_tiles(sW[0], sW[0] + 256, sW[0] + 512, lane, qn * BV_TILE, m, cs0, sCol[0],
+                             sRow[0], sT[0], sW[0], sPiv[0], &s_dotd[0], &s_info[0]);
+        bv_panel_logdet(lane, qn * BV_TILE, m, cs0, sPiv[0], &s_logdet[0]);
+        bv_wave_sync();
+        *(bv_d4 *)(ws + bv_tile_off(qn, qn, T) + 4 * lane) = *(const bv_d4 *)(sW[0] + 4 * lane);
+        *(bv_d4 *)(ws + bv_tile_off(qn + 1, qn, T) + 4 * lane) =
+            *(const bv_d4 *)(sW[0] + 256 + 4 * lane);
+        *(bv_d4 *)(ws + bv_tile_off(qn + 1, qn + 1, T) + 4 * lane) =
+            *(const bv_d4 *)(sW[0] + 512 + 4 * lane);
+    } else if (hasB) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                e[i][c] = -bv_pair_tile<KIND>(ws, T, qn + 2 + i, qn + c, m, (2 + i) * BV_TILE,
+                                              c * BV_TILE, lane, qx, qy, qz, qo, kp, planar);
+        }
+        if (s >= 0) {
+            bv_step_gemm<2, 2>(ws, T, qn / 2, c0, qn + 2, lane, e);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const bv_d4 la0 = *(const bv_d4 *)(sX + (0 * 4 + kk) * 256 + 4 * lane);
+                const bv_d4 la1 = *(const bv_d4 *)(sX + (1 * 4 + kk) * 256 + 4 * lane);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        e[i][0] = bv_mfma(la0[g], x[i][kk][g], e[i][0]);
+                        e[i][1] = bv_mfma(la1[g], x[i][kk][g], e[i][1]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads(); /* W_A in sW[0]; wave A's partials */
+    if (w == 1 && hasB) {
+        const bv_d4 wa00 = *(const bv_d4 *)(sW[0] + 4 * lane);
+        const bv_d4 wa10 = *(const bv_d4 *)(sW[0] + 256 + 4 * lane);
+        const bv_d4 wa11 = *(const bv_d4 *)(sW[0] + 512 + 4 * lane);
+        bv_d4 lba[2][2];
+        double dba = 0.0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            bv_solve_pair(-e[i][0], -e[i][1], wa00, wa10, wa11, lba[i][0], lba[i][1]);
+            const int rt = qn + 2 + i;
+            *(bv_d4 *)(ws + bv_tile_off(rt, qn, T) + 4 * lane) = lba[i][0];
+            *(bv_d4 *)(ws + bv_tile_off(rt, qn + 1, T) + 4 * lane) = lba[i][1];
+            dba += bv_row_m_sq<2>(lba[i], rt, qn * BV_TILE, m, cs0, lane);
+        }
+        dba = bv_wave_sum(dba);
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                db0 = bv_mfma(lba[0][kk][g], lba[0][kk][g], db0);
+                db1 = bv_mfma(lba[0][kk][g], lba[1][kk][g], db1);
+                db2 = bv_mfma(lba[1][kk][g], lba[1][kk][g], db2);
+            }
+        }
+        *(bv_d4 *)(sW[1] + 0 * 256 + 4 * lane) = -db0;
+        *(bv_d4 *)(sW[1] + 1 * 256 + 4 * lane) = -db1;
+        *(bv_d4 *)(sW[1] + 2 * 256 + 4 * lane) = -db2;
+        bv_wave_sync();
+        bv_diag_factor_tiles(sW[1], sW[1] + 256, sW[1] + 512, lane, (qn + 2) * BV_TILE, m, cs0,
+                             sCol[1], sRow[1], sT[1], sW[1], sPiv[1], &s_dotd[1], &s_info[1]);
+        bv_panel_logdet(lane, (qn + 2) * BV_TILE, m, cs0, sPiv[1], &s_logdet[1]);
+        bv_wave_sync();
+        *(bv_d4 *)(ws + bv_tile_off(qn + 2, qn + 2, T) + 4 * lane) =
+            *(const bv_d4 *)(sW[1] + 4 * lane);
+        *(bv_d4 *)(ws + bv_tile_off(qn + 3, qn + 2, T) + 4 * lane) =
+            *(const bv_d4 *)(sW[1] + 256 + 4 * lane);
+        *(bv_d4 *)(ws + bv_tile_off(qn + 3, qn + 3, T) + 4 * lane) =
+            *(const bv_d4 *)(sW[1] + 512 + 4 * lane);
+        if (lane == 0) {
+            const int64_t o = (int64_t)it.desc * pmax + pa;
+            part_logdet[o] = s_logdet[0];
+            part_dot[o] = s_dotd[0] + dba;
+            part_info[o] = s_info[0];
+            part_logdet[o + 1] = s_logdet[1];
+            part_dot[o + 1] = s_dotd[1];
+            part_info[o + 1] = s_info[1];
+        }
+    } else if (w == 0 && !hasB && lane == 0) {
+        const int64_t o = (int64_t)it.desc * pmax + pa;
+        part_logdet[o] = s_logdet[0];
+        part_dot[o] = s_dotd[0];
+        part_info[o] = s_info[0];
+    }
+}
+
 /* Per-block totals of the stepped factorization's partials, fixed order. */
 __global__ void __launch_bounds__(256)
 bv_step_parts_kernel(int ndesc, const bv_block_desc *__restrict__ desc,
@@ -1486,6 +1889,32 @@ hipError_t bv_launch_step2(int kind, unsigned nitems, int j, hipStream_t st,
     default:
         hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(64), 0, st, items,
                            (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t bv_launch_pair(int kind, unsigned nitems, int s, hipStream_t st,
+                          const bv_step_item *items, const bv_block_desc *desc,
+                          const bv_eval_slot *slots, double *ws, double *part_logdet,
+                          double *part_dot, int32_t *part_info, int pmax) {
+    if (nitems == 0) return hipSuccess;
+    switch (kind) {
+    case BV_KIND_NU05:
+        hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_NU05>, dim3(nitems), dim3(128), 0, st, items,
+                           (int)nitems, s, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    case BV_KIND_NU15:
+        hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_NU15>, dim3(nitems), dim3(128), 0, st, items,
+                           (int)nitems, s, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    case BV_KIND_NU25:
+        hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_NU25>, dim3(nitems), dim3(128), 0, st, items,
+                           (int)nitems, s, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    default:
+        hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(128), 0, st, items,
+                           (int)nitems, s, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     }
     return hipGetLastError();

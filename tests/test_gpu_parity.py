@@ -280,6 +280,8 @@ FACTOR_CONFIGS = {
     "stepped2": {"BV_PATH": "stepped2", "BV_CHUNKS": "1", "BV_CHUNK_STREAMS": "1"},
     "stepped2_3chunks_2streams": {"BV_PATH": "stepped2", "BV_CHUNKS": "3",
                                   "BV_CHUNK_STREAMS": "2"},
+    "paired": {"BV_PATH": "paired", "BV_CHUNKS": "1", "BV_CHUNK_STREAMS": "1"},
+    "paired_3chunks_2streams": {"BV_PATH": "paired", "BV_CHUNKS": "3", "BV_CHUNK_STREAMS": "2"},
 }
 
 
@@ -287,8 +289,8 @@ FACTOR_CONFIGS = {
 def test_factor_configs_match_oracle(c1_full, name, monkeypatch):
     """Every factorization configuration the autotuner can choose, fixed by
     environment (the fused kernel with four, one or two wavefronts per block;
-    the stepped factorization with and without lookahead, whole or chunked
-    over streams) against the
+    the stepped factorization with and without lookahead and by pairs of
+    panels, whole or chunked over streams) against the
     oracle: the n=2000 workload, a general-nu case and a large-block case."""
     for k, v in FACTOR_CONFIGS[name].items():
         monkeypatch.setenv(k, v)
