@@ -98,6 +98,7 @@ def calibration_copy(mib, repeats=3):
 
 
 FACTOR_PREFIXES = ("void bv_block_kernel<", "void bv_step_kernel<", "void bv_step2_kernel<",
+                   "void bv_pair_kernel<",
                    "void bv_block_kernel_1w<", "bv_step_parts_kernel",
                    "void bv_gen_kernel<")
 
@@ -341,6 +342,9 @@ def main():
     if cfg is None or cfg[1] == 0:
         nw = {0: "four wavefronts", 1: "one wavefront", 2: "two wavefronts"}[cfg[0] if cfg else 0]
         kname = "bv_block_kernel (%s, %s per block)" % (nu_desc, nw)
+    elif cfg[1] == 3:
+        kname = "bv_pair_kernel (paired panels, lookahead) x %d launches (%s, %d chunk(s)%s)" % (
+            (eng_pmax(blocks, cs) + 1) // 2, nu_desc, cfg[2], ", HIP graph replay" if cfg[4] else "")
     else:
         kname = "%s x %d panel-step launches (%s, %d chunk(s)%s)" % (
             "bv_step2_kernel (lookahead)" if cfg[1] == 2 else "bv_step_kernel",

@@ -30,15 +30,15 @@ hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t s
 hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,
                          const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
 hipError_t bv_launch_reduce(unsigned nevals, hipStream_t st, const bv_eval_slot *slots);
-hipError_t bv_launch_step(int kind, unsigned nitems, int j, hipStream_t st,
+hipError_t bv_launch_step(int kind, unsigned nitems, hipStream_t st,
                           const bv_step_item *items, const bv_block_desc *desc,
                           const bv_eval_slot *slots, double *ws, double *part_logdet,
                           double *part_dot, int32_t *part_info, int pmax);
-hipError_t bv_launch_step2(int kind, unsigned nitems, int j, hipStream_t st,
+hipError_t bv_launch_step2(int kind, unsigned nitems, hipStream_t st,
                            const bv_step_item *items, const bv_block_desc *desc,
                            const bv_eval_slot *slots, double *ws, double *part_logdet,
                            double *part_dot, int32_t *part_info, int pmax);
-hipError_t bv_launch_pair(int kind, unsigned nitems, int s, hipStream_t st,
+hipError_t bv_launch_pair(int kind, unsigned nitems, hipStream_t st,
                           const bv_step_item *items, const bv_block_desc *desc,
                           const bv_eval_slot *slots, double *ws, double *part_logdet,
                           double *part_dot, int32_t *part_info, int pmax);
@@ -193,60 +193,83 @@ static int bv_layout(bv_ctx *c, int k) {
        round-robin (workgroup w runs on XCD w % 8), so the items of one
        descriptor get ids of one residue mod 8: they share the panel-row and
        W tiles through that XCD's L2.  Descriptors go to the least loaded XCD;
-       shorter queues are padded with empty items. */
+       shorter queues are padded with empty items.  Items that factor a
+       diagonal block ahead (lookahead forms) are dispatched before every other
+       item of their launch, so that their serial factorizations start first.
+       With chunks skewed on one stream (nstreams == 0), launch L holds chunk
+       c's items of step L - c: the chunks' steps overlap inside each launch
+       instead of across streams. */
     const int NX = 8;
     std::vector<bv_step_item> items;
     const int NC = c->nchunks;
-    c->step_off.assign((size_t)NC * (c->pmax + 1), 0);
-    for (int ch = 0; ch < NC; ++ch) {
-        for (int j = 0; j < c->pmax; ++j) {
-            c->step_off[(size_t)ch * (c->pmax + 1) + j] = (int64_t)items.size();
-            /* qa: the items that factor a diagonal block ahead (lookahead
-               forms), dispatched before every other item of the launch so
-               that their serial factorizations start first; qb: the rest */
-            std::vector<std::vector<bv_step_item>> qa(NX), qb(NX);
-            for (size_t i = (size_t)ch; i < all.size(); i += (size_t)NC) {
-                const int T = all[i].mp / BV_TILE, P = all[i].mp / BV_NB;
-                /* stepped: launch j covers rows 2j.. of blocks with more than
-                   j panels; lookahead (stepped == 2): launch index j is panel
-                   step j - 1, covering rows 2j.. of blocks with more than j
-                   panels (index 0: the first diagonal block alone); paired
-                   (stepped == 3): launch index j is pair step j - 1, covering
-                   rows 4j.. in workgroups of four row tiles (index 0: the
-                   first pair's diagonal block alone) */
-                const int rows = (c->stepped == 3) ? 4 : BV_STEP_ROWS;
-                const int r0 = (c->stepped == 3) ? 4 * j : 2 * j;
-                if (c->stepped == 3 ? r0 >= T : j >= P) continue;
-                size_t x = 0;
-                for (int t = 1; t < NX; ++t)
-                    if (qa[t].size() + qb[t].size() < qa[x].size() + qb[x].size()) x = t;
-                const int rend = (c->stepped >= 2 && j == 0) ? std::min(c->stepped == 3 ? 4 : 2, T) : T;
-                for (int r = r0; r < rend; r += rows) {
-                    bv_step_item it;
-                    it.desc = (int32_t)i;
-                    it.row0 = r;
-                    it.nrows = std::min(rows, rend - r);
-                    it.pad_ = 0;
-                    const bool ahead = c->look_first && c->stepped >= 2 && r == r0;
-                    (ahead ? qa : qb)[x].push_back(it);
-                }
-            }
-            for (const std::vector<std::vector<bv_step_item>> *qs : {&qa, &qb}) {
-                size_t len = 0;
-                for (int t = 0; t < NX; ++t) len = std::max(len, (*qs)[t].size());
-                for (size_t w = 0; w < len * NX; ++w) {
-                    const std::vector<bv_step_item> &qq = (*qs)[w % NX];
-                    const size_t k2 = w / NX;
-                    if (k2 < qq.size()) items.push_back(qq[k2]);
-                    else {
-                        bv_step_item e;
-                        e.desc = -1; e.row0 = 0; e.nrows = 0; e.pad_ = 0;
-                        items.push_back(e);
-                    }
-                }
+    const bool skew = c->nstreams == 0 && NC > 1;
+    typedef std::vector<std::vector<bv_step_item>> bv_queues;
+    auto build = [&](int ch, int j, bv_queues &qa, bv_queues &qb) {
+        for (size_t i = (size_t)ch; i < all.size(); i += (size_t)NC) {
+            const int T = all[i].mp / BV_TILE, P = all[i].mp / BV_NB;
+            /* stepped: launch j covers rows 2j.. of blocks with more than
+               j panels; lookahead (stepped == 2): launch index j is panel
+               step j - 1, covering rows 2j.. of blocks with more than j
+               panels (index 0: the first diagonal block alone); paired
+               (stepped == 3): launch index j is pair step j - 1, covering
+               rows 4j.. in workgroups of four row tiles (index 0: the
+               first pair's diagonal block alone) */
+            const int rows = (c->stepped == 3) ? 4 : BV_STEP_ROWS;
+            const int r0 = (c->stepped == 3) ? 4 * j : 2 * j;
+            if (c->stepped == 3 ? r0 >= T : j >= P) continue;
+            size_t x = 0;
+            for (int t = 1; t < NX; ++t)
+                if (qa[t].size() + qb[t].size() < qa[x].size() + qb[x].size()) x = t;
+            const int rend =
+                (c->stepped >= 2 && j == 0) ? std::min(c->stepped == 3 ? 4 : 2, T) : T;
+            for (int r = r0; r < rend; r += rows) {
+                bv_step_item it;
+                it.desc = (int32_t)i;
+                it.row0 = r;
+                it.nrows = std::min(rows, rend - r);
+                it.step = (c->stepped >= 2) ? j - 1 : j;
+                const bool ahead = c->look_first && c->stepped >= 2 && r == r0;
+                (ahead ? qa : qb)[x].push_back(it);
             }
         }
-        c->step_off[(size_t)ch * (c->pmax + 1) + c->pmax] = (int64_t)items.size();
+    };
+    auto emit = [&](const bv_queues &q) {
+        size_t len = 0;
+        for (int t = 0; t < NX; ++t) len = std::max(len, q[t].size());
+        for (size_t w = 0; w < len * NX; ++w) {
+            const std::vector<bv_step_item> &qq = q[w % NX];
+            const size_t k2 = w / NX;
+            if (k2 < qq.size()) items.push_back(qq[k2]);
+            else {
+                bv_step_item e;
+                e.desc = -1; e.row0 = 0; e.nrows = 0; e.step = 0;
+                items.push_back(e);
+            }
+        }
+    };
+    c->step_off.assign((size_t)NC * (c->pmax + 1), 0);
+    if (skew) {
+        const int nl = c->pmax + NC - 1; /* <= NC (pmax + 1) - 1 offsets */
+        for (int L = 0; L < nl; ++L) {
+            c->step_off[L] = (int64_t)items.size();
+            bv_queues qa(NX), qb(NX);
+            for (int ch = 0; ch < NC; ++ch)
+                if (L - ch >= 0 && L - ch < c->pmax) build(ch, L - ch, qa, qb);
+            emit(qa);
+            emit(qb);
+        }
+        c->step_off[nl] = (int64_t)items.size();
+    } else {
+        for (int ch = 0; ch < NC; ++ch) {
+            for (int j = 0; j < c->pmax; ++j) {
+                c->step_off[(size_t)ch * (c->pmax + 1) + j] = (int64_t)items.size();
+                bv_queues qa(NX), qb(NX);
+                build(ch, j, qa, qb);
+                emit(qa);
+                emit(qb);
+            }
+            c->step_off[(size_t)ch * (c->pmax + 1) + c->pmax] = (int64_t)items.size();
+        }
     }
     if (c->d_items) (void)hipFree(c->d_items);
     c->d_items = nullptr;
@@ -341,7 +364,7 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
                      : (strcmp(v, "paired") == 0)   ? 3
                                                     : 0;
     if (const char *v = getenv("BV_CHUNKS")) c->nchunks = std::max(1, std::min(64, atoi(v)));
-    if (const char *v = getenv("BV_CHUNK_STREAMS")) c->nstreams = std::max(1, std::min(4, atoi(v)));
+    if (const char *v = getenv("BV_CHUNK_STREAMS")) c->nstreams = std::max(0, std::min(4, atoi(v)));
     if (getenv("BV_FACTOR_KERNEL") || getenv("BV_PATH") || getenv("BV_CHUNKS")) c->autotune = 0;
     if (const char *v = getenv("BV_AUTOTUNE")) c->autotune = c->autotune && atoi(v) != 0;
     if (const char *v = getenv("BV_GRAPH")) c->use_graph = atoi(v) != 0;
@@ -449,22 +472,26 @@ static int bv_enqueue(bv_ctx *c, unsigned nblk, int kinds_used) {
                gets its first launch at once, so the chunks' chains overlap
                from the start (on one stream, chunk by chunk was slower: the
                launches lose parallelism and gain no cache reuse) */
-            for (int j = 0; j < c->pmax; ++j) {
-                for (int ch = 0; ch < c->nchunks; ++ch) {
+            const bool skew = c->nstreams == 0 && c->nchunks > 1;
+            const int nl = skew ? c->pmax + c->nchunks - 1 : c->pmax;
+            const int nch = skew ? 1 : c->nchunks;
+            for (int j = 0; j < nl; ++j) {
+                for (int ch = 0; ch < nch; ++ch) {
                     hipStream_t st = side ? c->cstream[ch % c->nstreams] : c->stream;
                     const int64_t *off = c->step_off.data() + (size_t)ch * (c->pmax + 1);
                     const int64_t o = off[j], cnt = off[j + 1] - o;
                     if (cnt <= 0) continue;
+                    /* every item carries its own step */
                     if (c->stepped == 3)
-                        BV_HIP(bv_launch_pair(kind, (unsigned)cnt, j - 1, st, c->d_items + o,
+                        BV_HIP(bv_launch_pair(kind, (unsigned)cnt, st, c->d_items + o,
                                               c->d_desc, c->d_slots, c->d_ws, c->d_part_logdet,
                                               c->d_part_dot, c->d_part_info, c->pmax));
                     else if (c->stepped == 2)
-                        BV_HIP(bv_launch_step2(kind, (unsigned)cnt, j - 1, st, c->d_items + o,
+                        BV_HIP(bv_launch_step2(kind, (unsigned)cnt, st, c->d_items + o,
                                                c->d_desc, c->d_slots, c->d_ws, c->d_part_logdet,
                                                c->d_part_dot, c->d_part_info, c->pmax));
                     else
-                        BV_HIP(bv_launch_step(kind, (unsigned)cnt, j, st, c->d_items + o,
+                        BV_HIP(bv_launch_step(kind, (unsigned)cnt, st, c->d_items + o,
                                               c->d_desc, c->d_slots, c->d_ws, c->d_part_logdet,
                                               c->d_part_dot, c->d_part_info, c->pmax));
                 }
@@ -618,10 +645,13 @@ struct bv_cfg {
 };
 static const bv_cfg bv_cfgs[] = {{0, 0, 1, 1}, {1, 0, 1, 1}, {2, 0, 1, 1}, {0, 1, 1, 1},
                                  {0, 1, 2, 2}, {0, 1, 3, 3}, {0, 2, 2, 2}, {0, 2, 3, 3},
-                                 {0, 3, 1, 1}, {0, 3, 2, 2}, {0, 3, 3, 3}};
+                                 {0, 3, 1, 1}, {0, 3, 2, 2}, {0, 3, 3, 3}, {0, 2, 2, 0},
+                                 {0, 3, 2, 0}};
 
 static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
-    const bool relayout = g.nchunks != c->nchunks || g.stepped != c->stepped;
+    /* the step items depend on the chunks and on whether they are skewed */
+    const bool relayout = g.nchunks != c->nchunks || g.stepped != c->stepped ||
+                          (g.nstreams == 0) != (c->nstreams == 0);
     c->variant = g.variant;
     c->stepped = g.stepped;
     c->nchunks = g.nchunks;

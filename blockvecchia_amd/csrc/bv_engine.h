@@ -52,8 +52,9 @@ struct bv_block_desc {
 struct bv_step_item {
     int32_t desc;      /* index into the descriptor array, or -1          */
     int32_t row0;      /* first row tile (2j for the item holding the diagonal block) */
-    int32_t nrows;     /* row tiles in this item, 1..BV_STEP_ROWS          */
-    int32_t pad_;
+    int32_t nrows;     /* row tiles in this item, 1..BV_STEP_ROWS (paired: 1..4) */
+    int32_t step;      /* the item's panel step (paired: pair step); one launch
+                          may hold items of several steps of different blocks */
 };
 
 /* Everything one evaluation needs: dataset pointers, covariance parameters

@@ -710,7 +710,7 @@ __device__ __forceinline__ void bv_solve_pair(const bv_d4 &c0, const bv_d4 &c1, 
 
 template <int KIND>
 __global__ void __launch_bounds__(64, BV_STEP_WAVES_PER_SIMD)
-bv_step_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
+bv_step_kernel(const bv_step_item *__restrict__ items, int nitems,
                const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
                double *ws_base, double *part_logdet, double *part_dot, int32_t *part_info,
                int pmax) {
@@ -727,6 +727,7 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
     if (it.desc < 0) return;
+    const int j = it.step;
     const bv_block_desc d = desc[it.desc];
     const bv_eval_slot &E = slots[d.eval];
     const int lane = threadIdx.x;
@@ -960,7 +961,7 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
  * ====================================================================== */
 template <int KIND>
 __global__ void __launch_bounds__(64, BV_STEP_WAVES_PER_SIMD)
-bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
+bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
                 const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
                 double *ws_base, double *part_logdet, double *part_dot, int32_t *part_info,
                 int pmax) {
@@ -978,6 +979,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems, int j,
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
     if (it.desc < 0) return;
+    const int j = it.step;
     const bv_block_desc d = desc[it.desc];
     const bv_eval_slot &E = slots[d.eval];
     const int lane = threadIdx.x;
@@ -1321,7 +1323,7 @@ __device__ __forceinline__ double bv_wave_sum(double v) {
 
 template <int KIND>
 __global__ void __launch_bounds__(128, BV_PAIR_WAVES_PER_SIMD)
-bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems, int s,
+bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
                const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
                double *ws_base, double *part_logdet, double *part_dot, int32_t *part_info,
                int pmax) {
@@ -1337,6 +1339,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems, int s,
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
     if (it.desc < 0) return;
+    const int s = it.step;
     const bv_block_desc d = desc[it.desc];
     const bv_eval_slot &E = slots[d.eval];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -1842,7 +1845,7 @@ hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_
     return hipGetLastError();
 }
 
-hipError_t bv_launch_step(int kind, unsigned nitems, int j, hipStream_t st,
+hipError_t bv_launch_step(int kind, unsigned nitems, hipStream_t st,
                           const bv_step_item *items, const bv_block_desc *desc,
                           const bv_eval_slot *slots, double *ws, double *part_logdet,
                           double *part_dot, int32_t *part_info, int pmax) {
@@ -1850,25 +1853,25 @@ hipError_t bv_launch_step(int kind, unsigned nitems, int j, hipStream_t st,
     switch (kind) {
     case BV_KIND_NU05:
         hipLaunchKernelGGL(bv_step_kernel<BV_KIND_NU05>, dim3(nitems), dim3(64), 0, st, items,
-                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     case BV_KIND_NU15:
         hipLaunchKernelGGL(bv_step_kernel<BV_KIND_NU15>, dim3(nitems), dim3(64), 0, st, items,
-                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     case BV_KIND_NU25:
         hipLaunchKernelGGL(bv_step_kernel<BV_KIND_NU25>, dim3(nitems), dim3(64), 0, st, items,
-                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     default:
         hipLaunchKernelGGL(bv_step_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(64), 0, st, items,
-                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     }
     return hipGetLastError();
 }
 
-hipError_t bv_launch_step2(int kind, unsigned nitems, int j, hipStream_t st,
+hipError_t bv_launch_step2(int kind, unsigned nitems, hipStream_t st,
                           const bv_step_item *items, const bv_block_desc *desc,
                           const bv_eval_slot *slots, double *ws, double *part_logdet,
                           double *part_dot, int32_t *part_info, int pmax) {
@@ -1876,25 +1879,25 @@ hipError_t bv_launch_step2(int kind, unsigned nitems, int j, hipStream_t st,
     switch (kind) {
     case BV_KIND_NU05:
         hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_NU05>, dim3(nitems), dim3(64), 0, st, items,
-                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     case BV_KIND_NU15:
         hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_NU15>, dim3(nitems), dim3(64), 0, st, items,
-                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     case BV_KIND_NU25:
         hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_NU25>, dim3(nitems), dim3(64), 0, st, items,
-                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     default:
         hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(64), 0, st, items,
-                           (int)nitems, j, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     }
     return hipGetLastError();
 }
 
-hipError_t bv_launch_pair(int kind, unsigned nitems, int s, hipStream_t st,
+hipError_t bv_launch_pair(int kind, unsigned nitems, hipStream_t st,
                           const bv_step_item *items, const bv_block_desc *desc,
                           const bv_eval_slot *slots, double *ws, double *part_logdet,
                           double *part_dot, int32_t *part_info, int pmax) {
@@ -1902,19 +1905,19 @@ hipError_t bv_launch_pair(int kind, unsigned nitems, int s, hipStream_t st,
     switch (kind) {
     case BV_KIND_NU05:
         hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_NU05>, dim3(nitems), dim3(128), 0, st, items,
-                           (int)nitems, s, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     case BV_KIND_NU15:
         hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_NU15>, dim3(nitems), dim3(128), 0, st, items,
-                           (int)nitems, s, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     case BV_KIND_NU25:
         hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_NU25>, dim3(nitems), dim3(128), 0, st, items,
-                           (int)nitems, s, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     default:
         hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(128), 0, st, items,
-                           (int)nitems, s, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     }
     return hipGetLastError();

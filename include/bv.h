@@ -112,8 +112,11 @@ double bv_last_kernel_ms(bv_ctx *ctx);
  * wavefronts per block, 1: one, 2: two), cfg[1] = 1 for the stepped factorization
  * (one launch per panel step over all blocks), 2 for the stepped factorization
  * with lookahead (the next panel's diagonal block factored one launch
- * early), 0 for the fused kernel, cfg[2] = block chunks,
- * cfg[3] = streams, cfg[4] = 1 when single evaluations replay a HIP graph
+ * early), 3 for the paired-panel factorization (one launch per pair of
+ * panels, the next pair's 64x64 diagonal block factored one launch early),
+ * 0 for the fused kernel, cfg[2] = block chunks, cfg[3] = streams the chunks
+ * cycle over (0: one stream, chunk c's step j in the same launch as chunk
+ * 0's step j + c), cfg[4] = 1 when single evaluations replay a HIP graph
  * (cfg holds 5 entries).  No counterpart in the reference.  The first
  * evaluation of a context times the candidate configurations on its own data
  * and keeps the fastest, then times graph replay against direct launches for

@@ -18,6 +18,7 @@ CONFIGS = {
     "stepped_2chunks": {"BV_PATH": "stepped", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
     "stepped2_2chunks": {"BV_PATH": "stepped2", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
     "paired_2chunks": {"BV_PATH": "paired", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
+    "stepped2_2chunks_skewed": {"BV_PATH": "stepped2", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "0"},
 }
 
 

@@ -282,6 +282,10 @@ FACTOR_CONFIGS = {
                                   "BV_CHUNK_STREAMS": "2"},
     "paired": {"BV_PATH": "paired", "BV_CHUNKS": "1", "BV_CHUNK_STREAMS": "1"},
     "paired_3chunks_2streams": {"BV_PATH": "paired", "BV_CHUNKS": "3", "BV_CHUNK_STREAMS": "2"},
+    # chunks skewed on one stream: chunk c's step j runs in chunk 0's launch j + c
+    "stepped_2chunks_skewed": {"BV_PATH": "stepped", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "0"},
+    "stepped2_2chunks_skewed": {"BV_PATH": "stepped2", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "0"},
+    "paired_3chunks_skewed": {"BV_PATH": "paired", "BV_CHUNKS": "3", "BV_CHUNK_STREAMS": "0"},
 }
 
 
