@@ -1201,7 +1201,8 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
  * factors the next pair's 64x64 diagonal block D = A(Q,Q) - L(Q, <c0+4)
  * L(Q, <c0+4)^T: wave A's 2x2 tile block is accumulated with its pass (a),
  * wave B's own block likewise and its cross block D_BA by a second pass over
- * the columns < c0 while wave A factors D_AA (W_A); then wave B forms
+ * the columns < c0 (the pair's columns from wave A's L tiles, stored before a
+ * workgroup barrier) while wave A factors D_AA (W_A); then wave B forms
  * L_BA = D_BA W_A^T, D_BB - L_BA L_BA^T and factors it (W_B).  W_A, L_BA and
  * W_B go to the workspace with the panels' log-determinants, row-m squares
  * and first failures in the (block, panel) partials.  Launch -1 is that
@@ -1210,6 +1211,9 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
  * ====================================================================== */
 #ifndef BV_PAIR_WAVES_PER_SIMD
 #define BV_PAIR_WAVES_PER_SIMD 2
+#endif
+#ifndef BV_PAIR_PINGPONG
+#define BV_PAIR_PINGPONG 1 /* two operand buffers in the column loop */
 #endif
 
 /* The pair's four row tiles and two own row tiles of one tile column. */
@@ -1251,6 +1255,7 @@ __device__ __forceinline__ void bv_pair_gemm(const double *__restrict__ ws, int 
     if (kend <= 0) return;
     const int lo = 4 * lane;
     const double *colp = ws;
+#if BV_PAIR_PINGPONG
     bv_d4 pa[4], ra[2], pb[4], rb[2];
     bv_pair_load(colp, c0, rbase, lo, pa, ra);
     for (int kt = 0; kt < kend; kt += 2) {
@@ -1262,6 +1267,14 @@ __device__ __forceinline__ void bv_pair_gemm(const double *__restrict__ ws, int 
         bv_pair_step<SYRK>(pb, rb, acc, d00, d10, d11);
         colp = cola;
     }
+#else
+    for (int kt = 0; kt < kend; ++kt) {
+        bv_d4 pa[4], ra[2];
+        bv_pair_load(colp, c0, rbase, lo, pa, ra);
+        bv_pair_step<SYRK>(pa, ra, acc, d00, d10, d11);
+        colp += ((int64_t)(T - kt - 1) << 8);
+    }
+#endif
 }
 
 /* Tile (rt, ct) of the augmented matrix: generated from the points staged in
@@ -1315,6 +1328,25 @@ __device__ __forceinline__ double bv_row_m_sq(const bv_d4 (&x)[NC], int rt, int 
     return t;
 }
 
+/* d += L(r, pair) L(r', pair)^T for the own rows r, r' in {rbase, rbase + 1}
+   (lower 2x2 tile block), the pair's four tile columns read back from the
+   workspace where this wavefront stored them. */
+__device__ __forceinline__ void bv_pair_syrk_own(const double *__restrict__ ws, int T, int rbase,
+                                                 int c0, int lane, bv_d4 &d00, bv_d4 &d10,
+                                                 bv_d4 &d11) {
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const bv_d4 x0 = *(const bv_d4 *)(ws + bv_tile_off(rbase, c0 + kk, T) + 4 * lane);
+        const bv_d4 x1 = *(const bv_d4 *)(ws + bv_tile_off(rbase + 1, c0 + kk, T) + 4 * lane);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            d00 = bv_mfma(x0[g], x0[g], d00);
+            d10 = bv_mfma(x0[g], x1[g], d10);
+            d11 = bv_mfma(x1[g], x1[g], d11);
+        }
+    }
+}
+
 __device__ __forceinline__ double bv_wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -1328,7 +1360,6 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
                double *ws_base, double *part_logdet, double *part_dot, int32_t *part_info,
                int pmax) {
     __shared__ double qx[128], qy[128], qz[128], qo[128]; /* row points, then the pair's columns */
-    __shared__ double sX[8 * 256];                        /* wave A's L(Q_A, pair) for wave B */
     __shared__ double sW[2][3 * 256];                     /* each wave's diagonal block, W out */
     __shared__ double sT[2][16 * 17];
     __shared__ double sCol[2][16], sRow[2][16], sPiv[2][BV_NB];
@@ -1367,7 +1398,6 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
                         kp.metric == 0 && !kp.use_z;
     const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
 
-    bv_d4 x[2][4];                          /* L(own rows, pair) */
     bv_d4 d00 = zero, d10 = zero, d11 = zero; /* own rows' SYRK over k < c0 */
     if (s >= 0 && mine) {
         /* -A(R, pair), written out: the generation is too large for the
@@ -1382,6 +1412,14 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
         if (look) bv_pair_gemm<true>(ws, T, c0, c0, myr, lane, acc, d00, d10, d11);
         else bv_pair_gemm<false>(ws, T, c0, c0, myr, lane, acc, d00, d10, d11);
         const int lo = 4 * lane;
+        bv_d4 x[2][2]; /* L(own rows, one panel of the pair) */
+        bool has_m = false;
+        double dp0 = 0.0, dp1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int rt = myr + i;
+            if (rt * BV_TILE <= m && m < (rt + 1) * BV_TILE) has_m = true;
+        }
         {
             const bv_d4 w00 = *(const bv_d4 *)(ws + bv_tile_off(c0, c0, T) + lo);
             const bv_d4 w10 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 1, c0, T) + lo);
@@ -1391,7 +1429,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
                 bv_solve_pair(-acc[i][0], -acc[i][1], w00, w10, w11, x[i][0], x[i][1]);
         }
         {
-            /* C(R, 2s+1) -= L(R, 2s) L(panel 2s+1 rows, 2s)^T */
+            /* C(R, 2s+1) -= L(R, 2s) L(panel 2s+1 rows, 2s)^T, then L(R, 2s) is stored */
             const bv_d4 l00 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 2, c0, T) + lo);
             const bv_d4 l01 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 2, c0 + 1, T) + lo);
             const bv_d4 l10 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 3, c0, T) + lo);
@@ -1406,28 +1444,26 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
                     acc[i][3] = bv_mfma(l11[g], x[i][1][g], acc[i][3]);
                 }
             }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int rt = myr + i;
+                *(bv_d4 *)(ws + bv_tile_off(rt, c0, T) + lo) = x[i][0];
+                *(bv_d4 *)(ws + bv_tile_off(rt, c0 + 1, T) + lo) = x[i][1];
+                dp0 += bv_row_m_sq<2>(x[i], rt, c0 * BV_TILE, m, cs0, lane);
+            }
         }
         {
             const bv_d4 w00 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 2, c0 + 2, T) + lo);
             const bv_d4 w10 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 3, c0 + 2, T) + lo);
             const bv_d4 w11 = *(const bv_d4 *)(ws + bv_tile_off(c0 + 3, c0 + 3, T) + lo);
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-                bv_solve_pair(-acc[i][2], -acc[i][3], w00, w10, w11, x[i][2], x[i][3]);
-        }
-        bool has_m = false;
-        double dp0 = 0.0, dp1 = 0.0;
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int rt = myr + i;
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                *(bv_d4 *)(ws + bv_tile_off(rt, c0 + c, T) + lo) = x[i][c];
-            if (rt * BV_TILE <= m && m < (rt + 1) * BV_TILE) has_m = true;
-            const bv_d4 xa[2] = {x[i][0], x[i][1]};
-            const bv_d4 xb[2] = {x[i][2], x[i][3]};
-            dp0 += bv_row_m_sq<2>(xa, rt, c0 * BV_TILE, m, cs0, lane);
-            dp1 += bv_row_m_sq<2>(xb, rt, (c0 + 2) * BV_TILE, m, cs0, lane);
+            for (int i = 0; i < 2; ++i) {
+                const int rt = myr + i;
+                bv_solve_pair(-acc[i][2], -acc[i][3], w00, w10, w11, x[i][0], x[i][1]);
+                *(bv_d4 *)(ws + bv_tile_off(rt, c0 + 2, T) + lo) = x[i][0];
+                *(bv_d4 *)(ws + bv_tile_off(rt, c0 + 3, T) + lo) = x[i][1];
+                dp1 += bv_row_m_sq<2>(x[i], rt, (c0 + 2) * BV_TILE, m, cs0, lane);
+            }
         }
         if (has_m) {
             dp0 = bv_wave_sum(dp0);
@@ -1453,25 +1489,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
                                              planar);
         bv_d4 da2 = d11 - bv_pair_tile<KIND>(ws, T, qn + 1, qn + 1, m, 16, 16, lane, qx, qy, qz, qo,
                                              kp, planar);
-        if (s >= 0) {
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    da0 = bv_mfma(x[0][kk][g], x[0][kk][g], da0);
-                    da1 = bv_mfma(x[0][kk][g], x[1][kk][g], da1);
-                    da2 = bv_mfma(x[1][kk][g], x[1][kk][g], da2);
-                }
-            }
-            if (hasB) {
-#pragma unroll
-                for (int i = 0; i < 2; ++i) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        *(bv_d4 *)(sX + (i * 4 + c) * 256 + 4 * lane) = x[i][c];
-                }
-            }
-        }
+        if (s >= 0) bv_pair_syrk_own(ws, T, myr, c0, lane, da0, da1, da2);
         *(bv_d4 *)(sW[0] + 0 * 256 + 4 * lane) = -da0;
         *(bv_d4 *)(sW[0] + 1 * 256 + 4 * lane) = -da1;
         *(bv_d4 *)(sW[0] + 2 * 256 + 4 * lane) = -da2;
@@ -1483,20 +1501,10 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
                                        planar);
         db2 = d11 - bv_pair_tile<KIND>(ws, T, qn + 3, qn + 3, m, 48, 48, lane, qx, qy, qz, qo, kp,
                                        planar);
-        if (s >= 0) {
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    db0 = bv_mfma(x[0][kk][g], x[0][kk][g], db0);
-                    db1 = bv_mfma(x[0][kk][g], x[1][kk][g], db1);
-                    db2 = bv_mfma(x[1][kk][g], x[1][kk][g], db2);
-                }
-            }
-        }
+        if (s >= 0) bv_pair_syrk_own(ws, T, myr, c0, lane, db0, db1, db2);
         if (lane == 0) { s_logdet[1] = 0.0; s_dotd[1] = 0.0; s_info[1] = 0; }
     }
-    __syncthreads(); /* sX, sW[0] */
+    __syncthreads(); /* sW[0]; wave A's L(Q_A, pair) tiles in the workspace */
     if (w == 0) {
         bv_diag_factor_tiles(sW[0], sW[0] + 256, sW[0] + 512, lane, qn * BV_TILE, m, cs0, sCol[0],
                              sRow[0], sT[0], sW[0], sPiv[0], &s_dotd[0], &s_info[0]);
@@ -1519,15 +1527,17 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
             bv_step_gemm<2, 2>(ws, T, qn / 2, c0, qn + 2, lane, e);
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) {
-                const bv_d4 la0 = *(const bv_d4 *)(sX + (0 * 4 + kk) * 256 + 4 * lane);
-                const bv_d4 la1 = *(const bv_d4 *)(sX + (1 * 4 + kk) * 256 + 4 * lane);
+                /* stored by wave A before the barrier */
+                const bv_d4 la0 = *(const bv_d4 *)(ws + bv_tile_off(qn, c0 + kk, T) + 4 * lane);
+                const bv_d4 la1 = *(const bv_d4 *)(ws + bv_tile_off(qn + 1, c0 + kk, T) + 4 * lane);
+                const bv_d4 lb0 = *(const bv_d4 *)(ws + bv_tile_off(qn + 2, c0 + kk, T) + 4 * lane);
+                const bv_d4 lb1 = *(const bv_d4 *)(ws + bv_tile_off(qn + 3, c0 + kk, T) + 4 * lane);
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
-#pragma unroll
-                    for (int i = 0; i < 2; ++i) {
-                        e[i][0] = bv_mfma(la0[g], x[i][kk][g], e[i][0]);
-                        e[i][1] = bv_mfma(la1[g], x[i][kk][g], e[i][1]);
-                    }
+                    e[0][0] = bv_mfma(la0[g], lb0[g], e[0][0]);
+                    e[0][1] = bv_mfma(la1[g], lb0[g], e[0][1]);
+                    e[1][0] = bv_mfma(la0[g], lb1[g], e[1][0]);
+                    e[1][1] = bv_mfma(la1[g], lb1[g], e[1][1]);
                 }
             }
         }
