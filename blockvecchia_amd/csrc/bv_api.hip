@@ -103,6 +103,7 @@ struct bv_ctx {
     std::vector<int64_t> step_off;          /* [chunk][pmax + 1] item offsets      */
     int nchunks = 1, nstreams = 1;          /* block chunks, and streams they cycle over */
     int look_first = 1;                     /* lookahead items dispatched first */
+    int chunk_split = 1;                    /* chunks: contiguous by cost (1) or round-robin */
     int autotune = 1, tuned = 0;            /* choose the configuration on first use */
     hipStream_t cstream[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t cev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -204,8 +205,26 @@ static int bv_layout(bv_ctx *c, int k) {
     const int NC = c->nchunks;
     const bool skew = c->nstreams == 0 && NC > 1;
     typedef std::vector<std::vector<bv_step_item>> bv_queues;
+    /* chunk of each descriptor: contiguous ranges of equal factorization cost
+       (Mp^3) over the size-sorted list, so the blocks with the most panels
+       share one chunk and only its stream runs the extra steps at the end;
+       or round-robin (BV_CHUNK_SPLIT=0) */
+    std::vector<int> chunk_of(all.size(), 0);
+    {
+        double tot = 0.0, acc = 0.0;
+        for (const bv_block_desc &d : all) tot += (double)d.mp * d.mp * d.mp;
+        for (size_t i = 0; i < all.size(); ++i) {
+            const double cst = (double)all[i].mp * all[i].mp * all[i].mp;
+            if (c->chunk_split)
+                chunk_of[i] = std::min(NC - 1, (int)((acc + 0.5 * cst) / tot * NC));
+            else
+                chunk_of[i] = (int)(i % (size_t)NC);
+            acc += cst;
+        }
+    }
     auto build = [&](int ch, int j, bv_queues &qa, bv_queues &qb) {
-        for (size_t i = (size_t)ch; i < all.size(); i += (size_t)NC) {
+        for (size_t i = 0; i < all.size(); ++i) {
+            if (chunk_of[i] != ch) continue;
             const int T = all[i].mp / BV_TILE, P = all[i].mp / BV_NB;
             /* stepped: launch j covers rows 2j.. of blocks with more than
                j panels; lookahead (stepped == 2): launch index j is panel
@@ -369,6 +388,7 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     if (const char *v = getenv("BV_AUTOTUNE")) c->autotune = c->autotune && atoi(v) != 0;
     if (const char *v = getenv("BV_GRAPH")) c->use_graph = atoi(v) != 0;
     if (const char *v = getenv("BV_LOOK_FIRST")) c->look_first = atoi(v) != 0;
+    if (const char *v = getenv("BV_CHUNK_SPLIT")) c->chunk_split = atoi(v) != 0;
     c->h_batch_num.assign(p->batch_num, p->batch_num + p->batch_count);
     c->h_accum.assign((size_t)c->B + 1, 0);
     for (int i = 0; i < c->B; ++i) {
