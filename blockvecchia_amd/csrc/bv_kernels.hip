@@ -304,9 +304,86 @@ __device__ __forceinline__ bv_d4 bv_transpose16(const bv_d4 &v, int lane, double
    global index of local column 0.  The squared pivots a_kk = l_kk^2 go to
    sPiv[pivoff + k]; squares of row m's factor entries in columns >= cs0 add to
    dacc; the first failing column (global, + 1) to info. */
+#ifndef BV_SWEEP_DPP
+#define BV_SWEEP_DPP 1
+#endif
+
+/* The value of v in lane K of this lane's row of 16 lanes (DPP row_newbcast,
+   no LDS round trip). */
+template <int K>
+__device__ __forceinline__ double bv_row_bcast(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x150 + K, 0xF, 0xF, false);
+    const int hi =
+        __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x150 + K, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+/* Step K of the sweep below without LDS: the column-K factors of the other
+   rows that the update needs are taken from row K (the trailing block is
+   exactly symmetric: A and every product subtracted from it are, entry for
+   entry; the observation row's column, where the augmented matrix is not
+   symmetric, only meets pivots of 1 and zero factors), broadcast within each
+   16-lane row by DPP, as is row K of W.  Only the row's own column-K factor
+   crosses rows (one lane permutation). */
+template <int K>
+__device__ __forceinline__ void bv_sweep_step(bv_d4 &a, bv_d4 &w, int lane, int kbase, int m,
+                                              int cs0, double *sPiv, int pivoff, double &dacc,
+                                              int &info) {
+    constexpr int kq = K & 3, kg = K >> 2;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int kml = m - kbase;
+    const int q = kbase + K;
+    const double akk = bv_readlane(a[kg], K + 16 * kq);
+    double pinv = 1.0;
+    if (q < m) {
+        const double ha = 0.5 * akk;
+        double y = __builtin_amdgcn_rsq(akk);
+        y = fma(y, fma(-ha * y, y, 0.5), y);
+        y = fma(y, fma(-ha * y, y, 0.5), y);
+        pinv = y;
+    }
+    if (q < m && !(akk > 0.0) && info == 0) info = q + 1;
+    const double t = a[kg] * pinv;
+    const double lrk = __shfl(t, lr + 16 * kq, 64);
+    if (lr == kml && K < kml && q >= cs0 && lq == 0) dacc += lrk * lrk;
+    if (lane == 0) sPiv[pivoff + K] = akk; /* l_kk^2: log-determinant term */
+    bv_d4 lk, wk;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        lk[g] = bv_row_bcast<K>(a[g]) * pinv;
+        wk[g] = bv_row_bcast<K>(w[g]) * pinv;
+    }
+    const double lre = (lr > K) ? lrk : 0.0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        a[g] -= lrk * lk[g];
+        w[g] = (lr == K) ? wk[g] : w[g] - lre * wk[g];
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void bv_sweep_steps(bv_d4 &a, bv_d4 &w, int lane, int kbase, int m,
+                                               int cs0, double *sPiv, int pivoff, double &dacc,
+                                               int &info) {
+    bv_sweep_step<K>(a, w, lane, kbase, m, cs0, sPiv, pivoff, dacc, info);
+    if constexpr (K + 1 < 16)
+        bv_sweep_steps<K + 1>(a, w, lane, kbase, m, cs0, sPiv, pivoff, dacc, info);
+}
+
 __device__ __forceinline__ bv_d4 bv_sweep16(bv_d4 a, int lane, int kbase, int m, int cs0,
                                             double *sCol, double *sRow, double *sPiv,
                                             int pivoff, double &dacc, int &info) {
+#if BV_SWEEP_DPP
+    (void)sCol;
+    (void)sRow;
+    const int lr0 = lane & 15, lq0 = lane >> 4;
+    bv_d4 w0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) w0[g] = (4 * g + lq0 == lr0) ? 1.0 : 0.0;
+    bv_sweep_steps<0>(a, w0, lane, kbase, m, cs0, sPiv, pivoff, dacc, info);
+    return w0;
+#endif
     const int lr = lane & 15, lq = lane >> 4;
     const int kml = m - kbase;
     bv_d4 w;
