@@ -68,15 +68,17 @@ def traffic_from_profiles(config):
     copy-kernel calibration), or None when it was measured for another
     factorization configuration than the one in use."""
     p = os.path.join(ROOT, "profiles", "factorization_n20000_cs300_traffic.json")
-    if not os.path.exists(p):
+    if not os.path.exists(p) or config is None:
         return None
     with open(p) as f:
         d = json.load(f)
-    # the kernels (and so their traffic) depend on the first four entries
-    # only; the fifth says whether they are replayed from a graph
-    if config is None or list(d.get("config", []))[:4] != list(config)[:4]:
-        return None
-    return d.get("hbm_bytes_per_launch")
+    # one entry per measured configuration; the kernels (and so their
+    # traffic) depend on the first four entries of a configuration only, the
+    # fifth says whether they are replayed from a graph
+    for e in d.get("entries", [d]):
+        if list(e.get("config", []))[:4] == list(config)[:4]:
+            return e.get("hbm_bytes_per_launch")
+    return None
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "factorization_n20000_cs300_traffic.json")
@@ -145,8 +147,15 @@ def summarize_traffic(fetch_csv, write_csv, mib, config):
         "hbm_fetch_bytes_per_launch": fetch, "hbm_write_bytes_per_launch": write,
         "hbm_bytes_per_launch": fetch + write,
     }
+    entries = []
+    if os.path.exists(TRAFFIC_JSON):
+        with open(TRAFFIC_JSON) as fh:
+            old = json.load(fh)
+        entries = [e for e in old.get("entries", [old])
+                   if list(e.get("config", []))[:4] != list(config)[:4]]
+    entries.append(d)
     with open(TRAFFIC_JSON, "w") as fh:
-        json.dump(d, fh, indent=1)
+        json.dump({"entries": entries}, fh, indent=1)
     print(json.dumps(d, indent=1))
 
 
