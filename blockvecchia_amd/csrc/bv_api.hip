@@ -103,6 +103,7 @@ struct bv_ctx {
     std::vector<int64_t> step_off;          /* [chunk][pmax + 1] item offsets      */
     int nchunks = 1, nstreams = 1;          /* block chunks, and streams they cycle over */
     int look_first = 1;                     /* lookahead items dispatched first */
+    int srv_busy = 0;                       /* a server lane runs an evaluation on it */
     int chunk_split = 1;                    /* chunks: contiguous by cost (1) or round-robin */
     int autotune = 1, tuned = 0;            /* choose the configuration on first use */
     hipStream_t cstream[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -904,6 +905,8 @@ struct bv_lane {
     hipEvent_t done = nullptr;
     bool busy = false;
     std::vector<bv_request> reqs;            /* the batch in flight          */
+    bv_ctx *own = nullptr;                   /* set: one evaluation on its dataset's own
+                                                tuned path, results in h_result */
     /* capacities (grown while the lane is idle) */
     int64_t cap_ws = 0, cap_blocks = 0;
     int cap_evals = 0;
@@ -922,6 +925,11 @@ struct bv_server {
        kernel: merged batches hold many blocks, and one wavefront per block
        keeps more of them resident (measured faster at every size tried) */
     int64_t one_wave_max_mp = INT32_MAX;
+    /* an evaluation whose dataset's workspace is at least this many bytes,
+       and whose autotuned configuration is a stepped form, fills the GPU on
+       its own: it runs alone on its dataset's tuned path (stream, items,
+       configuration) instead of in a merged fused launch */
+    double own_min_bytes = 2e8;
     std::vector<bv_ctx *> datasets;
     std::vector<bv_request> queue;           /* FIFO (front = oldest)        */
     size_t qhead = 0;
@@ -981,6 +989,7 @@ extern "C" int bv_server_create(int device, int nlanes, double ws_bytes_per_lane
     s->ws_budget = ws_bytes_per_lane;
     s->max_evals = max_evals_per_batch;
     if (const char *v = getenv("BV_SERVER_1W_MAXMP")) s->one_wave_max_mp = atoi(v);
+    if (const char *v = getenv("BV_SERVER_OWN_BYTES")) s->own_min_bytes = atof(v);
     s->lanes.resize(nlanes);
     for (bv_lane &L : s->lanes) {
         if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1035,12 +1044,67 @@ extern "C" int bv_server_submit(bv_server *s, int dataset_id, int nparam, const 
 }
 
 /* Admit queued requests into the idle lane L and launch them. */
+static bool bv_server_own(const bv_server *s, const bv_ctx *c) {
+    return c->stepped > 0 && (c->tuned || !c->autotune) &&
+           8.0 * (double)c->ws_per_eval >= s->own_min_bytes;
+}
+
+/* One evaluation alone on its dataset's tuned path, enqueued on the dataset's
+   stream without waiting (as bv_run_once, with the lane's pinned slot and
+   result buffers so that other calls on the dataset cannot touch them). */
+static int bv_server_launch_own(bv_server *s, bv_lane &L, const bv_request &r) {
+    bv_ctx *c = s->datasets[r.dataset];
+    int rc;
+    if ((rc = bv_lane_reserve(L, 0, 0, 1))) return rc;
+    if ((rc = bv_reserve(c, 1)) || (rc = bv_layout(c, 1))) return rc;
+    bv_eval_slot &sl = L.h_slots[0];
+    memset(&sl, 0, sizeof(sl));
+    if ((rc = bv_make_kparams(c, r.theta, &sl.kp, &sl.kind))) return rc;
+    sl.x = c->d_x; sl.y = c->d_y; sl.z = c->d_z; sl.obs = c->d_obs;
+    sl.nbr = c->d_nbr; sl.cx = c->d_cx; sl.cy = c->d_cy; sl.cz = c->d_cz; sl.cobs = c->d_cobs;
+    sl.cs = c->cs;
+    sl.nblocks = c->B;
+    sl.batch_num = c->d_batch_num;
+    sl.logdet = c->d_logdet;
+    sl.dot = c->d_dot;
+    sl.info = c->d_info;
+    sl.result = c->d_result;
+    sl.log2pi = log(2 * 3.141592653589793);
+    BV_HIP(hipSetDevice(s->device));
+    BV_HIP(hipMemcpyAsync(c->d_slots, L.h_slots, sizeof(bv_eval_slot), hipMemcpyHostToDevice,
+                          c->stream));
+    const int kinds_used = 1 << sl.kind;
+    rc = c->use_graph && c->graph_wins ? bv_enqueue_graph(c, (unsigned)c->B, kinds_used)
+                                       : bv_enqueue(c, (unsigned)c->B, kinds_used);
+    if (rc) return rc;
+    BV_HIP(bv_launch_reduce(1u, c->stream, c->d_slots));
+    BV_HIP(hipMemcpyAsync(L.h_result, c->d_result, sizeof(double) * 4, hipMemcpyDeviceToHost,
+                          c->stream));
+    BV_HIP(hipEventRecord(L.done, c->stream));
+    c->srv_busy = 1;
+    L.own = c;
+    L.reqs.push_back(r);
+    L.busy = true;
+    return BV_OK;
+}
+
 static int bv_server_launch(bv_server *s, bv_lane &L) {
     L.reqs.clear();
+    L.own = nullptr;
+    if (s->qhead < s->queue.size()) {
+        const bv_request &r = s->queue[s->qhead];
+        bv_ctx *c = s->datasets[r.dataset];
+        if (bv_server_own(s, c)) {
+            if (c->srv_busy) return BV_OK; /* its dataset's path is in use: wait */
+            ++s->qhead;
+            return bv_server_launch_own(s, L, r);
+        }
+    }
     int64_t ws = 0, blocks = 0;
     while (s->qhead < s->queue.size() && (int)L.reqs.size() < s->max_evals) {
         const bv_request &r = s->queue[s->qhead];
         const bv_ctx *c = s->datasets[r.dataset];
+        if (bv_server_own(s, c)) break; /* runs alone on its own path */
         const double bytes = 8.0 * (double)(ws + c->ws_per_eval);
         if (!L.reqs.empty() && bytes > s->ws_budget) break; /* admission by workspace bytes */
         ws += c->ws_per_eval;
@@ -1142,6 +1206,8 @@ extern "C" int bv_server_poll(bv_server *s, int wait, int max_out, int64_t *tags
             }
             L.busy = false;
             L.reqs.clear();
+            if (L.own) L.own->srv_busy = 0;
+            L.own = nullptr;
         }
         /* start queued work on idle lanes */
         for (bv_lane &L : s->lanes) {
@@ -1171,6 +1237,8 @@ extern "C" void bv_server_destroy(bv_server *s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
     for (bv_lane &L : s->lanes) {
+        if (L.busy && L.done) (void)hipEventSynchronize(L.done); /* own-path work too */
+        if (L.own) L.own->srv_busy = 0;
         if (L.stream) (void)hipStreamSynchronize(L.stream);
         bv_lane_free(L);
         if (L.done) (void)hipEventDestroy(L.done);

@@ -254,6 +254,20 @@ def test_llh_n20000_cs300(c2_blocks):
     _check(c2_blocks, [1.5, 0.1, 0.5])
 
 
+@pytest.mark.parametrize("name", ["stepped2", "paired_2chunks_skewed", "paired_2chunks"])
+def test_llh_n20000_cs300_configs(c2_blocks, name, monkeypatch):
+    """The two configurations the autotuner picks between for the benchmark
+    workload, fixed by environment, against the oracle at full size."""
+    env = {"stepped2": {"BV_PATH": "stepped2", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
+           "paired_2chunks_skewed": {"BV_PATH": "paired", "BV_CHUNKS": "2",
+                                     "BV_CHUNK_STREAMS": "0"},
+           "paired_2chunks": {"BV_PATH": "paired", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"}}
+    for k, v in env[name].items():
+        monkeypatch.setenv(k, v)
+    _say("case: n=20000 bc=1500 cs=300 nu=0.5 %s" % name)
+    _check(c2_blocks, [1.5, 0.1, 0.5])
+
+
 def test_n20000_scaling_law(c2_blocks):
     """Size-independent property at full size: llh(c s2) from the s2 = 1
     per-block terms."""

@@ -104,3 +104,49 @@ def test_server_large_request_runs_alone_and_rejects_bad_theta():
     srv.close()
     for e in engines:
         e.close()
+
+
+def test_server_own_path_requests(monkeypatch):
+    """Evaluations whose dataset's tuned path is a stepped form and whose
+    workspace passes the size threshold run alone on that path (the
+    dataset's stream, items and configuration) instead of in a merged fused
+    launch; at most one at a time per dataset, the rest wait in the queue.
+    Results equal bv_llh's bit for bit (the same kernels), mixed with merged
+    requests of a small dataset."""
+    monkeypatch.setenv("BV_PATH", "stepped2")
+    monkeypatch.setenv("BV_CHUNKS", "2")
+    monkeypatch.setenv("BV_CHUNK_STREAMS", "2")
+    big = [bv.Engine(b) for b in _datasets()[2:]]
+    monkeypatch.setenv("BV_PATH", "fused")
+    small = bv.Engine(_datasets()[0])
+    monkeypatch.setenv("BV_SERVER_OWN_BYTES", "1")
+    engines = big + [small]
+    reqs = []
+    tag = 0
+    for rep in range(4):
+        for d in range(len(engines)):
+            reqs.append((tag, d, [1.5 * (1 + 0.02 * rep), 0.1 * (1 - 0.01 * rep), 0.5]))
+            tag += 1
+    alone = {t: engines[d].llh(th) for t, d, th in reqs}
+    srv = bv.Server(lanes=3, ws_bytes_per_lane=3e7, max_evals_per_batch=4)
+    ids = [srv.attach(e) for e in engines]
+    _say("own path: %d requests over %d datasets" % (len(reqs), len(engines)))
+    got = {}
+    for t, d, th in reqs:
+        srv.submit(ids[d], th, t)
+        for tt, v, st in srv.poll(wait=False):
+            got[tt] = (v, st)
+    while srv.pending() > 0:
+        for tt, v, st in srv.poll(wait=True):
+            got[tt] = (v, st)
+    srv.close()
+    assert sorted(got) == sorted(alone)
+    for t, d, th in reqs:
+        v, st = got[t]
+        assert st == bv.BV_OK
+        if d < len(big):
+            assert v == alone[t], (t, v, alone[t])
+        else:
+            assert abs(v - alone[t]) <= LLH_RTOL * abs(alone[t]), (t, v, alone[t])
+    for e in engines:
+        e.close()
