@@ -238,6 +238,33 @@ def run_uneven(args):
         "per_nu": out}))
 
 
+def run_serving_c5(args):
+    """Serving: 32 datasets (n from {5k, 20k, 50k, 200k}, cs from {30, 60,
+    150, 300}, n/13 blocks, nu from {1/2, 3/2, 5/2}), 4,096 requests near each
+    dataset's truth arriving by Poisson at 0.6 times the server's capacity;
+    served against plain on the same arrival stream.  One JSON line."""
+    from blockvecchia_amd import serving as srv
+    r = srv.benchmark_c5(log=log)
+    log("serving c5 (open loop): %.0f evals/s vs plain %.0f (x%.2f), p50 %.2f ms p99 %.2f ms, "
+        "plain p50 %.2f ms p99 %.2f ms; closed loop %.0f vs %.0f evals/s (x%.2f); max rel "
+        "diff %.2e" % (
+            r["evals_per_s"], r["plain_evals_per_s"], r["speedup_vs_plain"],
+            r["latency_ms"]["p50"], r["latency_ms"]["p99"], r["plain_latency_ms"]["p50"],
+            r["plain_latency_ms"]["p99"], r["closed_loop"]["evals_per_s"],
+            r["closed_loop"]["plain_evals_per_s"], r["closed_loop"]["speedup_vs_plain"],
+            r["max_rel_diff_vs_alone"]))
+    print(json.dumps({
+        "metric": METRIC, "value": r["evals_per_s"], "unit": "evals/s", "n_gpus": 1,
+        "steps": r["requests"], "warmup": 512, "ms_per_step": 1e3 / r["evals_per_s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (model draws: dense for n <= 20k, Block Vecchia above)",
+        "config": {"workload": "serving: 32 datasets, n in {5k,20k,50k,200k}, cs in "
+                               "{30,60,150,300}, bc=n/13, nu in {1/2,3/2,5/2}; 4096 Poisson "
+                               "requests at 0.6x server capacity",
+                   "parallelism": "single-gpu"},
+        "latency_ms": r["latency_ms"], "serving": r}), flush=True)
+
+
 def run_mle(args):
     """Full maximum-likelihood run with general smoothness: n = 100,000
     uniform, truth (1.0, 0.08, 0.8), observations from the Block Vecchia
@@ -304,7 +331,7 @@ def main():
                     help="append a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 counter CSVs into the traffic JSON and exit")
-    ap.add_argument("--workload", choices=["perf", "uneven", "mle"], default="perf",
+    ap.add_argument("--workload", choices=["perf", "uneven", "mle", "serving"], default="perf",
                     help="perf: the n=20000 cs=300 evaluation (default); uneven: n=200000 "
                          "clustered, cs=400, nu 3/2 and 5/2; mle: the driver's full "
                          "maximum-likelihood run at n=100000, general nu")
@@ -327,6 +354,8 @@ def main():
         sys.exit(2)
     if args.workload == "uneven":
         return run_uneven(args)
+    if args.workload == "serving":
+        return run_serving_c5(args)
     if args.workload == "mle":
         return run_mle(args)
 

@@ -241,6 +241,99 @@ def benchmark(sessions=32, per_session=16, lanes=4, ws_bytes=2 << 30, max_evals=
     }
 
 
+def build_c5_datasets(count=32, seed=5, log=None):
+    """The many-dataset serving configuration: `count` datasets with n drawn
+    from {5k, 20k, 50k, 200k} uniform on [0,1]^2, cs from {30, 60, 150,
+    300}, n/13 k-means blocks, Matern nu from {1/2, 3/2, 5/2}, sigma^2 = 1.5,
+    range 0.1.  Observations are model draws: the dense FP64 Cholesky (on the
+    GPU) for n <= 20k, the Block Vecchia model of the dataset's own partition
+    (sequential simulation on the host) for n >= 50k.  White noise would make
+    the quadratic forms so ill-conditioned that two orders of the same FP64
+    operations disagree far beyond rounding."""
+    import torch
+    device = "cuda" if torch.cuda.is_available() else None
+    rng = np.random.default_rng(seed)
+    specs, engines, truths = [], [], []
+    for i in range(count):
+        n = int(rng.choice([5000, 20000, 50000, 200000]))
+        cs = int(rng.choice([30, 60, 150, 300]))
+        nu = float(rng.choice([0.5, 1.5, 2.5]))
+        x, y = datagen.uniform_locations(n, seed + 100 + i)
+        if n <= 20000:
+            obs = datagen.sample_dense(x, y, 1.5, 0.1, nu, seed=seed + i, device=device)
+            blocks = bv.partition(x, y, obs, max(1, n // 13), cs, seed=0,
+                                  device=0 if device else None)
+        else:
+            blocks = bv.partition(x, y, np.zeros(n), max(1, n // 13), cs, seed=0,
+                                  device=0 if device else None)
+            blocks.obs = np.ascontiguousarray(
+                datagen.sample_block_vecchia(blocks, 1.5, 0.1, nu, seed=seed + i, threads=16),
+                dtype=np.float64)
+        engines.append(bv.Engine(blocks))
+        truths.append(np.array([1.5, 0.1, nu]))
+        specs.append((n, cs, nu))
+        if log:
+            log("c5 dataset %d: n=%d cs=%d nu=%g" % (i, n, cs, nu))
+    return specs, engines, truths
+
+
+def benchmark_c5(count=4096, load=0.6, lanes=4, seed=5, log=None):
+    """Open-loop serving of the many-dataset configuration: 32 datasets
+    (build_c5_datasets), `count` requests with a theta near each dataset's
+    truth arriving by Poisson at `load` times the server's closed-loop
+    capacity, served and, on the same arrival stream, plain (one evaluation
+    at a time); evaluations/s and p50/p99 latency from arrival."""
+    t0 = time.perf_counter()
+    specs, engines, truths = build_c5_datasets(seed=seed, log=log)
+    if log:
+        log("c5 datasets: %.1f s" % (time.perf_counter() - t0))
+    for e, th in zip(engines, truths):
+        e.llh(th)                                         # autotune each dataset
+    plan = make_requests(truths, 32, 8)
+    run_served(engines, plan, lanes)
+    _, t_cap, lat_cl = run_served(engines, plan, lanes)
+    _, t_pcl, lat_pcl = run_plain(engines, plan)
+    nplan = sum(len(ths) for _, ths in plan)
+    capacity = nplan / t_cap
+    rate = load * capacity
+    reqs = poisson_requests(truths, count, rate, seed)
+    run_open_served(engines, reqs[:512], lanes)          # warm-up
+    served, t_s, lat_s = run_open_served(engines, reqs, lanes)
+    if log:
+        log("c5 served: %.1f s" % t_s)
+    plain, t_p, lat_p = run_open_plain(engines, reqs)
+    if log:
+        log("c5 plain: %.1f s" % t_p)
+    worst = max(abs(served[k] - plain[k]) / abs(plain[k]) for k in plain)
+    # per dataset: the served and plain results of one request come from
+    # different kernels (merged one-wavefront launches or the dataset's tuned
+    # path), so they differ by the conditioning of that dataset's likelihood
+    per_ds = {}
+    for k in plain:
+        d = reqs[k][1]
+        per_ds[d] = max(per_ds.get(d, 0.0), abs(served[k] - plain[k]) / abs(plain[k]))
+    for e in engines:
+        e.close()
+    return {
+        "datasets": [{"n": n, "cs": cs, "nu": nu, "bc": max(1, n // 13)} for n, cs, nu in specs],
+        "requests": count, "lanes": lanes,
+        "offered_rate_per_s": rate, "server_capacity_per_s": capacity,
+        "evals_per_s": count / t_s, "plain_evals_per_s": count / t_p,
+        "speedup_vs_plain": t_p / t_s,
+        "latency_ms": {"p50": float(np.percentile(lat_s, 50) * 1e3),
+                       "p99": float(np.percentile(lat_s, 99) * 1e3)},
+        "plain_latency_ms": {"p50": float(np.percentile(lat_p, 50) * 1e3),
+                             "p99": float(np.percentile(lat_p, 99) * 1e3)},
+        "max_rel_diff_vs_alone": worst,
+        "max_rel_diff_by_dataset": [per_ds.get(d, 0.0) for d in range(len(specs))],
+        "closed_loop": {"sessions": 32, "requests": nplan,
+                        "evals_per_s": capacity, "plain_evals_per_s": nplan / t_pcl,
+                        "speedup_vs_plain": t_pcl / t_cap,
+                        "latency_ms": {"p50": float(np.percentile(lat_cl, 50) * 1e3),
+                                       "p99": float(np.percentile(lat_cl, 99) * 1e3)}},
+    }
+
+
 def scenarios():
     """The two serving scenarios reported by bench.py."""
     return {
