@@ -348,9 +348,9 @@ def main():
                           cfg)
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the likelihood shards by blocks "
-              "across GPUs with a single 8-byte sum; see DESIGN.md); --gpus %d is not run"
-              % args.gpus, file=sys.stderr)
+        print("bench.py: one GPU only (the likelihood would shard by blocks across GPUs, "
+              "cost-balanced contiguous ranges, with a single 8-byte sum at the end); "
+              "--gpus %d is not run" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == "uneven":
         return run_uneven(args)

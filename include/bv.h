@@ -233,7 +233,7 @@ int bv_partition_gpu(int32_t device, int64_t n, const double *x, const double *y
 
 void bv_blocks_free(bv_blocks *b);
 
-/* Checks that this build's matrix-core lane layout assumption holds on the
+/* Checks that the library's matrix-core lane layout assumption holds on the
  * current device (one 16x16x4 FP64 product against a host product). */
 int bv_device_selftest(int device);
 
