@@ -149,8 +149,12 @@ void bv_destroy(bv_ctx *ctx);
  * merges their blocks into shared launches, and returns finished results.
  * Merged launches factor with one wavefront per block (BV_SERVER_1W_MAXMP=M
  * in the environment at creation limits that to kind groups of padded size
- * <= M; larger ones use four wavefronts).  Calls on one server are not
- * thread-safe; a result does not depend on the batch it ran in. */
+ * <= M; larger ones use four wavefronts).  A request whose dataset is tuned
+ * to a stepped form and needs at least BV_SERVER_OWN_BYTES (default 2e8) of
+ * workspace runs alone on the dataset's own tuned path and stream, one at a
+ * time per dataset; it counts as the context's latest evaluation for
+ * bv_block_terms.  Calls on one server are not thread-safe; a merged
+ * result does not depend on the batch it ran in. */
 typedef struct bv_server bv_server;
 
 /* nlanes concurrent batches (1..16), each with ws_bytes_per_lane of
