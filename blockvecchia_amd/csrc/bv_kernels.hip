@@ -708,6 +708,9 @@ bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *_
 #ifndef BV_STEP_WAVES_PER_SIMD
 #define BV_STEP_WAVES_PER_SIMD 3
 #endif
+#ifndef BV_LOOK_PRIO
+#define BV_LOOK_PRIO 1 /* lookahead items at raised wave priority (0: off) */
+#endif
 
 /* acc += L(rows, k) L(panel rows, k)^T over tile columns [0, kend) from the
    workspace, for C consecutive row tiles rbase + s (ping-pong operands). */
@@ -1066,6 +1069,11 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
     const int jn = j + 1;                         /* the panel this launch factors */
     const int rn0 = 2 * jn;                       /* its diagonal block's first row tile */
     const bool look = (row0 == rn0) && (jn < P) && nr >= 2;
+#if BV_LOOK_PRIO
+    /* the lookahead item is the launch's longest serial chain: its
+       instructions issue first on its SIMD */
+    if (look) __builtin_amdgcn_s_setprio(3);
+#endif
     double *ws = ws_base + d.ws_off;
     const int lr = lane & 15, lq = lane >> 4;
 
@@ -1456,6 +1464,9 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
     const int row0 = it.row0, nr = it.nrows;
     const int c0 = 4 * s;
     const bool look = row0 == 4 * s + 4; /* this workgroup holds the next pair's rows */
+#if BV_LOOK_PRIO
+    if (look) __builtin_amdgcn_s_setprio(3); /* the launch's longest serial chain */
+#endif
     const int myr = row0 + 2 * w;
     const bool mine = 2 * w < nr;
     double *ws = ws_base + d.ws_off;
