@@ -226,6 +226,27 @@ def test_many_equals_single_and_deterministic(c1_small):
         eng.close()
 
 
+@pytest.mark.parametrize("name", ["stepped2_2chunks_skewed", "paired", "paired_3chunks_2streams",
+                                  "paired_3chunks_skewed"])
+def test_many_equals_single_stepped_forms(c1_small, name, monkeypatch):
+    """k evaluations in one call (descriptors of all of them interleaved in
+    every step launch, chunks over streams or skewed) give each evaluation's
+    bv_llh result bit for bit."""
+    for k, v in FACTOR_CONFIGS[name].items():
+        monkeypatch.setenv(k, v)
+    _say("case: llh_many vs llh, %s" % name)
+    x, y, obs = c1_small
+    eng = bv.Engine(bv.partition(x, y, obs, 30, 100, seed=0))
+    try:
+        thetas = np.array([[1.5, 0.1, 0.5], [1.2, 0.08, 0.5], [0.9, 0.2, 0.5]])
+        single = np.array([eng.llh(t) for t in thetas])
+        many, st = eng.llh_many(thetas)
+        assert np.all(st == 0)
+        np.testing.assert_array_equal(single, many)
+    finally:
+        eng.close()
+
+
 def test_not_positive_definite_status():
     _say("case: non-PD block reported")
     x = np.array([0.1, 0.1, 0.5, 0.7, 0.2, 0.9])
