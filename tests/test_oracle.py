@@ -74,6 +74,78 @@ def test_matern_zero_distance_and_earth():
     assert abs(A[0, 0] - math.exp(-d / 100.0)) < 1e-12
 
 
+def test_matern_3d_distance():
+    """--3D (use_z): the Euclidean distance over x, y, z, checked with numpy's
+    norm through the nu = 1/2 closed form sigma2 exp(-|a - b| / beta)."""
+    rng = np.random.default_rng(11)
+    P, Q = rng.random((17, 3)), rng.random((13, 3))
+    A = oracle.matern(P[:, 0], P[:, 1], Q[:, 0], Q[:, 1], [1.4, 0.3, 0.5], z1=P[:, 2],
+                      z2=Q[:, 2])
+    d = np.linalg.norm(P[:, None, :] - Q[None, :, :], axis=2)
+    np.testing.assert_allclose(A, 1.4 * np.exp(-d / 0.3), rtol=2e-15, atol=0)
+    # without z the same points give the planar distance: the z term matters
+    A2 = oracle.matern(P[:, 0], P[:, 1], Q[:, 0], Q[:, 1], [1.4, 0.3, 0.5])
+    assert np.max(np.abs(A2 - A)) > 1e-3
+
+
+def _st(P, Q, theta):
+    return oracle.matern(P[:, 0], P[:, 1], Q[:, 0], Q[:, 1], theta, kernel=4, z1=P[:, 2],
+                         z2=Q[:, 2])
+
+
+def test_spacetime_equal_times_is_spatial_matern():
+    """Kernel 4 with dt = 0: both temporal factors are 1, so the entry is the
+    spatial Matern of the (x, y) distance.  Closed forms at nu = 1/2 and 3/2
+    with numpy distances; a dropped factor or a wrong exponent in the
+    temporal terms would still pass here, the tests below catch those."""
+    rng = np.random.default_rng(12)
+    P, Q = rng.random((9, 3)), rng.random((11, 3))
+    P[:, 2] = 0.25
+    Q[:, 2] = 0.25
+    d = np.linalg.norm(P[:, None, :2] - Q[None, :, :2], axis=2)
+    for nu, f in [(0.5, lambda e: np.exp(-e)), (1.5, lambda e: (1 + e) * np.exp(-e))]:
+        th = [1.7, 0.2, nu, 0.8, 0.35, 0.6, 0.4]
+        np.testing.assert_allclose(_st(P, Q, th), 1.7 * f(d / 0.2), rtol=4e-15, atol=0)
+
+
+def test_spacetime_no_separability_terms_is_3d_matern():
+    """Kernel 4 with theta5 = theta6 = 0: (1 + |dt|^{2a}/c)^0 = 1, so at any
+    dt the entry is the Matern of the full x, y, z distance (the reference's
+    distance in this kernel includes z)."""
+    rng = np.random.default_rng(13)
+    P, Q = rng.random((8, 3)), rng.random((10, 3))
+    d = np.linalg.norm(P[:, None, :] - Q[None, :, :], axis=2)
+    th = [0.9, 0.25, 0.5, 0.6, 0.7, 0.0, 0.0]
+    np.testing.assert_allclose(_st(P, Q, th), 0.9 * np.exp(-d / 0.25), rtol=4e-15, atol=0)
+
+
+def test_spacetime_entry_by_hand():
+    """One entry with dt != 0 against a value computed at 40 digits with
+    mpmath from the Gneiting-type form C = s2 psi^-(b + d) M_nu(h / (beta
+    psi^(b/2))), psi = 1 + |dt|^(2 alpha) / a, h the x, y, z distance,
+    M_{3/2}(u) = (1 + u) e^-u: a = (0.1, 0.2, 0), b = (0.3, 0.25, 0.5),
+    theta = (1.3, 0.2, 1.5, 0.7, 0.45, 0.6, 0.3)."""
+    P = np.array([[0.1, 0.2, 0.0]])
+    Q = np.array([[0.3, 0.25, 0.5]])
+    v = _st(P, Q, [1.3, 0.2, 1.5, 0.7, 0.45, 0.6, 0.3])[0, 0]
+    assert abs(v - 0.2614444787599308693036548) <= 2e-15 * 0.2614
+    # symmetric in the two points, and sigma2 at zero distance and dt = 0
+    assert _st(Q, P, [1.3, 0.2, 1.5, 0.7, 0.45, 0.6, 0.3])[0, 0] == v
+    assert _st(P, P, [1.3, 0.2, 1.5, 0.7, 0.45, 0.6, 0.3])[0, 0] == 1.3
+
+
+def test_spacetime_zero_space_distance():
+    """Zero distance with dt != 0 cannot happen (the distance includes z);
+    the temporal decay alone: same x, y, different z.  psi^-(b + d) times the
+    Matern of |dt| / (beta psi^(b/2)) at nu = 1/2."""
+    P = np.array([[0.4, 0.4, 0.0]])
+    Q = np.array([[0.4, 0.4, 0.3]])
+    th = [2.0, 0.5, 0.5, 1.5, 0.5, 0.8, 0.2]
+    psi = 1 + 0.3 ** 1.0 / 1.5
+    ref = 2.0 * math.exp(-0.3 / (0.5 * psi ** 0.4)) / psi ** 1.0
+    assert abs(_st(P, Q, th)[0, 0] - ref) <= 4e-15 * ref
+
+
 def _blocks_from(x, y, obs, sizes, nbr, cs):
     return Blocks(x, y, obs, np.array(sizes, np.int32), np.array(nbr, np.int32).ravel(), cs)
 
