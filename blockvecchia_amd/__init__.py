@@ -82,7 +82,7 @@ EXPORTED = [
     "bv_partition", "bv_blocks_free", "bv_device_selftest",
     "bv_server_create", "bv_server_attach", "bv_server_submit", "bv_server_poll",
     "bv_server_pending", "bv_server_destroy", "bv_factor_config", "bv_set_graph",
-    "bv_partition_gpu",
+    "bv_partition_gpu", "bv_factor_candidates",
 ]
 
 _lib = None
@@ -133,6 +133,8 @@ def load_library(path=None):
     lib.bv_factor_config.restype = ctypes.c_int
     lib.bv_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.bv_set_graph.restype = ctypes.c_int
+    lib.bv_factor_candidates.argtypes = [_ip, ctypes.c_int]
+    lib.bv_factor_candidates.restype = ctypes.c_int
     lib.bv_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_void_p)]
     lib.bv_server_create.restype = ctypes.c_int
@@ -392,3 +394,13 @@ def matern(x1, y1, x2, y2, theta, kernel=1, metric=0, dist_scale=1.0, z1=None, z
 
 def device_selftest(device=0):
     return load_library().bv_device_selftest(device)
+
+
+def factor_candidates():
+    """The autotuner's candidate configurations, as (variant, stepped,
+    chunks, streams) tuples (bv_factor_candidates)."""
+    lib = load_library()
+    n = lib.bv_factor_candidates(None, 0)
+    buf = np.zeros(4 * n, np.int32)
+    lib.bv_factor_candidates(buf.ctypes.data_as(_ip), n)
+    return [tuple(int(v) for v in buf[4 * i:4 * i + 4]) for i in range(n)]

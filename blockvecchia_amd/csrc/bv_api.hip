@@ -14,7 +14,11 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "bv.h"
@@ -106,6 +110,7 @@ struct bv_ctx {
     int srv_busy = 0;                       /* a server lane runs an evaluation on it */
     int chunk_split = 1;                    /* chunks: contiguous by cost (1) or round-robin */
     int autotune = 1, tuned = 0;            /* choose the configuration on first use */
+    int fused_ok = 1;                       /* every block fits the fused kernels */
     hipStream_t cstream[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t cev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int pmax = 0;
@@ -174,49 +179,37 @@ static int bv_reserve(bv_ctx *c, int k) {
     return BV_OK;
 }
 
-/* Lay out the descriptors of k evaluations, interleaved so that the largest
-   blocks of every evaluation start first. */
-static int bv_layout(bv_ctx *c, int k) {
-    if (c->desc_evals == k) return BV_OK;
-    std::vector<bv_block_desc> all;
-    all.reserve((size_t)c->B * k);
-    for (const bv_block_desc &d0 : c->base_desc)
-        for (int e = 0; e < k; ++e) {
-            bv_block_desc d = d0;
-            d.eval = e;
-            d.ws_off = d0.ws_off + (int64_t)e * c->ws_per_eval;
-            all.push_back(d);
-        }
-    BV_HIP(hipMemcpyAsync(c->d_desc, all.data(), sizeof(bv_block_desc) * all.size(),
-                          hipMemcpyHostToDevice, c->stream));
-    /* stepped factorization: per step j, for every descriptor with more than
-       j panels, its rows 2j..T-1 in items of BV_STEP_ROWS row tiles (the item
-       holding the diagonal block first).  Workgroups are dealt to the XCDs
-       round-robin (workgroup w runs on XCD w % 8), so the items of one
-       descriptor get ids of one residue mod 8: they share the panel-row and
-       W tiles through that XCD's L2.  Descriptors go to the least loaded XCD;
-       shorter queues are padded with empty items.  Items that factor a
-       diagonal block ahead (lookahead forms) are dispatched before every other
-       item of their launch, so that their serial factorizations start first.
-       With chunks skewed on one stream (nstreams == 0), launch L holds chunk
-       c's items of step L - c: the chunks' steps overlap inside each launch
-       instead of across streams. */
+/* Work items of the stepped factorization `stepped` (1: one launch per panel
+   step, 2: with lookahead, 3: paired panels) over the descriptors all[0..nd).
+   Per step j, for every descriptor with more than j panels, its rows 2j..T-1
+   in items of BV_STEP_ROWS row tiles (paired: rows 4j.. in items of four; the
+   lookahead forms' launch index 0 holds the first diagonal block alone).
+   Workgroups are dealt to the XCDs round-robin (workgroup w runs on XCD
+   w % 8), so the items of one descriptor get ids of one residue mod 8: they
+   share the panel-row and W tiles through that XCD's L2.  Descriptors go to
+   the least loaded XCD; shorter queues are padded with empty items.  Items
+   that factor a diagonal block ahead (lookahead forms) are dispatched before
+   every other item of their launch when look_first is set, so that their
+   serial factorizations start first.  Chunks: contiguous ranges of equal
+   factorization cost (Mp^3) over the size-sorted list (chunk_split) or
+   round-robin; with skew, launch L holds chunk c's items of step L - c (the
+   chunks' steps overlap inside each launch).  step_off[ch * (pmax + 1) + j]
+   is the first item of chunk ch's launch j (skewed: step_off[L]).  Which
+   launch or stream runs an item does not change its arithmetic: a block's
+   result is the same under any chunking. */
+static void bv_build_items(const bv_block_desc *all, size_t nd, int stepped, int NC, bool skew,
+                           int chunk_split, int look_first, int pmax,
+                           std::vector<bv_step_item> &items, std::vector<int64_t> &step_off) {
     const int NX = 8;
-    std::vector<bv_step_item> items;
-    const int NC = c->nchunks;
-    const bool skew = c->nstreams == 0 && NC > 1;
     typedef std::vector<std::vector<bv_step_item>> bv_queues;
-    /* chunk of each descriptor: contiguous ranges of equal factorization cost
-       (Mp^3) over the size-sorted list, so the blocks with the most panels
-       share one chunk and only its stream runs the extra steps at the end;
-       or round-robin (BV_CHUNK_SPLIT=0) */
-    std::vector<int> chunk_of(all.size(), 0);
+    items.clear();
+    std::vector<int> chunk_of(nd, 0);
     {
         double tot = 0.0, acc = 0.0;
-        for (const bv_block_desc &d : all) tot += (double)d.mp * d.mp * d.mp;
-        for (size_t i = 0; i < all.size(); ++i) {
+        for (size_t i = 0; i < nd; ++i) tot += (double)all[i].mp * all[i].mp * all[i].mp;
+        for (size_t i = 0; i < nd; ++i) {
             const double cst = (double)all[i].mp * all[i].mp * all[i].mp;
-            if (c->chunk_split)
+            if (chunk_split)
                 chunk_of[i] = std::min(NC - 1, (int)((acc + 0.5 * cst) / tot * NC));
             else
                 chunk_of[i] = (int)(i % (size_t)NC);
@@ -224,31 +217,23 @@ static int bv_layout(bv_ctx *c, int k) {
         }
     }
     auto build = [&](int ch, int j, bv_queues &qa, bv_queues &qb) {
-        for (size_t i = 0; i < all.size(); ++i) {
+        for (size_t i = 0; i < nd; ++i) {
             if (chunk_of[i] != ch) continue;
             const int T = all[i].mp / BV_TILE, P = all[i].mp / BV_NB;
-            /* stepped: launch j covers rows 2j.. of blocks with more than
-               j panels; lookahead (stepped == 2): launch index j is panel
-               step j - 1, covering rows 2j.. of blocks with more than j
-               panels (index 0: the first diagonal block alone); paired
-               (stepped == 3): launch index j is pair step j - 1, covering
-               rows 4j.. in workgroups of four row tiles (index 0: the
-               first pair's diagonal block alone) */
-            const int rows = (c->stepped == 3) ? 4 : BV_STEP_ROWS;
-            const int r0 = (c->stepped == 3) ? 4 * j : 2 * j;
-            if (c->stepped == 3 ? r0 >= T : j >= P) continue;
+            const int rows = (stepped == 3) ? 4 : BV_STEP_ROWS;
+            const int r0 = (stepped == 3) ? 4 * j : 2 * j;
+            if (stepped == 3 ? r0 >= T : j >= P) continue;
             size_t x = 0;
             for (int t = 1; t < NX; ++t)
                 if (qa[t].size() + qb[t].size() < qa[x].size() + qb[x].size()) x = t;
-            const int rend =
-                (c->stepped >= 2 && j == 0) ? std::min(c->stepped == 3 ? 4 : 2, T) : T;
+            const int rend = (stepped >= 2 && j == 0) ? std::min(stepped == 3 ? 4 : 2, T) : T;
             for (int r = r0; r < rend; r += rows) {
                 bv_step_item it;
                 it.desc = (int32_t)i;
                 it.row0 = r;
                 it.nrows = std::min(rows, rend - r);
-                it.step = (c->stepped >= 2) ? j - 1 : j;
-                const bool ahead = c->look_first && c->stepped >= 2 && r == r0;
+                it.step = (stepped >= 2) ? j - 1 : j;
+                const bool ahead = look_first && stepped >= 2 && r == r0;
                 (ahead ? qa : qb)[x].push_back(it);
             }
         }
@@ -267,30 +252,52 @@ static int bv_layout(bv_ctx *c, int k) {
             }
         }
     };
-    c->step_off.assign((size_t)NC * (c->pmax + 1), 0);
+    step_off.assign((size_t)NC * (pmax + 1), 0);
     if (skew) {
-        const int nl = c->pmax + NC - 1; /* <= NC (pmax + 1) - 1 offsets */
+        const int nl = pmax + NC - 1; /* <= NC (pmax + 1) - 1 offsets */
         for (int L = 0; L < nl; ++L) {
-            c->step_off[L] = (int64_t)items.size();
+            step_off[L] = (int64_t)items.size();
             bv_queues qa(NX), qb(NX);
             for (int ch = 0; ch < NC; ++ch)
-                if (L - ch >= 0 && L - ch < c->pmax) build(ch, L - ch, qa, qb);
+                if (L - ch >= 0 && L - ch < pmax) build(ch, L - ch, qa, qb);
             emit(qa);
             emit(qb);
         }
-        c->step_off[nl] = (int64_t)items.size();
+        step_off[nl] = (int64_t)items.size();
     } else {
         for (int ch = 0; ch < NC; ++ch) {
-            for (int j = 0; j < c->pmax; ++j) {
-                c->step_off[(size_t)ch * (c->pmax + 1) + j] = (int64_t)items.size();
+            for (int j = 0; j < pmax; ++j) {
+                step_off[(size_t)ch * (pmax + 1) + j] = (int64_t)items.size();
                 bv_queues qa(NX), qb(NX);
                 build(ch, j, qa, qb);
                 emit(qa);
                 emit(qb);
             }
-            c->step_off[(size_t)ch * (c->pmax + 1) + c->pmax] = (int64_t)items.size();
+            step_off[(size_t)ch * (pmax + 1) + pmax] = (int64_t)items.size();
         }
     }
+}
+
+/* Lay out the descriptors of k evaluations, interleaved so that the largest
+   blocks of every evaluation start first, and the step items of the
+   configured stepped form (bv_build_items). */
+static int bv_layout(bv_ctx *c, int k) {
+    if (c->desc_evals == k) return BV_OK;
+    std::vector<bv_block_desc> all;
+    all.reserve((size_t)c->B * k);
+    for (const bv_block_desc &d0 : c->base_desc)
+        for (int e = 0; e < k; ++e) {
+            bv_block_desc d = d0;
+            d.eval = e;
+            d.ws_off = d0.ws_off + (int64_t)e * c->ws_per_eval;
+            all.push_back(d);
+        }
+    BV_HIP(hipMemcpyAsync(c->d_desc, all.data(), sizeof(bv_block_desc) * all.size(),
+                          hipMemcpyHostToDevice, c->stream));
+    std::vector<bv_step_item> items;
+    const bool skew = c->nstreams == 0 && c->nchunks > 1;
+    bv_build_items(all.data(), all.size(), c->stepped, c->nchunks, skew, c->chunk_split,
+                   c->look_first, c->pmax, items, c->step_off);
     if (c->d_items) (void)hipFree(c->d_items);
     c->d_items = nullptr;
     if (!items.empty()) {
@@ -418,7 +425,20 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
         off += T * (T + 1) / 2 * 256;
         c->max_mp = std::max(c->max_mp, d.mp);
     }
-    if (c->max_mp > BV_MAX_MP) { delete c; return BV_ERR_UNSUPPORTED; }
+    /* the fused kernels stage a block's points in LDS (4 Mp doubles): they
+       take padded sizes up to BV_MAX_MP; the stepped forms have no limit */
+    c->fused_ok = c->max_mp <= BV_MAX_MP;
+    if (!c->fused_ok && c->stepped == 0) {
+        if (getenv("BV_FACTOR_KERNEL") || getenv("BV_PATH")) {
+            fprintf(stderr,
+                    "libbv: a block's padded augmented size %d exceeds %d, the limit of the "
+                    "fused factorization kernels; use a stepped form (BV_PATH=stepped2)\n",
+                    c->max_mp, BV_MAX_MP);
+            delete c;
+            return BV_ERR_UNSUPPORTED;
+        }
+        c->stepped = 2; /* default without autotuning */
+    }
     c->ws_per_eval = off;
     std::stable_sort(desc.begin(), desc.end(),
                      [](const bv_block_desc &a, const bv_block_desc &b) { return a.mp > b.mp; });
@@ -682,16 +702,120 @@ static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
     return BV_OK;
 }
 
-/* bv_run_once, choosing the configuration on the first call: each candidate
-   runs once to warm up and twice timed (device time of the factorization),
-   the fastest is kept for the life of the context, and the call's results
-   come from a final run with it. */
+/* ---- autotune cache ----------------------------------------------------
+   The choice of the autotuner is kept in a text file, one line per workload:
+   "<key> <candidate index> <graph>", the last line of a key winning.  The key
+   names the device (architecture, compute units), the dataset's shape (block
+   count, conditioning size, kernel, metric, 3-D flag, a hash of every
+   block's padded size), the evaluations per call, the covariance kind and
+   the candidate list.  Path: BV_TUNE_CACHE, else
+   $HOME/.cache/blockvecchia_amd/autotune.txt; BV_TUNE_CACHE=0 or empty
+   disables it.  A context that finds its key skips the timing runs, so the
+   configuration (and with it the last bits of the results) is the same in
+   every process on the machine. */
+static std::string bv_tune_cache_path() {
+    const char *v = getenv("BV_TUNE_CACHE");
+    if (v) return (v[0] == 0 || strcmp(v, "0") == 0) ? std::string() : std::string(v);
+    const char *h = getenv("HOME");
+    if (!h || !h[0]) return std::string();
+    return std::string(h) + "/.cache/blockvecchia_amd/autotune.txt";
+}
+
+static std::string bv_tune_key(const bv_ctx *c, int k, int kinds_used) {
+    hipDeviceProp_t prop;
+    char arch[64] = "unknown";
+    int cus = 0;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
+        snprintf(arch, sizeof(arch), "%s", prop.gcnArchName);
+        for (char *q = arch; *q; ++q)
+            if (*q == ' ' || *q == ':') *q = '_';
+        cus = prop.multiProcessorCount;
+    }
+    uint64_t h = 1469598103934665603ULL; /* FNV-1a over the padded sizes */
+    for (const bv_block_desc &d : c->base_desc) {
+        h ^= (uint64_t)(uint32_t)d.mp;
+        h *= 1099511628211ULL;
+    }
+    for (const bv_cfg &g : bv_cfgs) {
+        h ^= (uint64_t)(g.variant | g.stepped << 4 | g.nchunks << 8 | g.nstreams << 16);
+        h *= 1099511628211ULL;
+    }
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%s/%d/B%d/cs%d/k%d/m%d/t%d/e%d/kind%d/%016llx", arch, cus, c->B,
+             c->cs, c->kernel, c->metric, c->time_flag, k, kinds_used, (unsigned long long)h);
+    return std::string(buf);
+}
+
+static bool bv_tune_cache_get(const std::string &path, const std::string &key, int *idx,
+                              int *graph) {
+    if (path.empty()) return false;
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[512], kbuf[300];
+    bool found = false;
+    while (fgets(line, sizeof(line), f)) {
+        int i = -1, g = 0;
+        if (sscanf(line, "%299s %d %d", kbuf, &i, &g) == 3 && key == kbuf) {
+            *idx = i;
+            *graph = g;
+            found = true;
+        }
+    }
+    fclose(f);
+    return found;
+}
+
+static void bv_tune_cache_put(const std::string &path, const std::string &key, int idx,
+                              int graph) {
+    if (path.empty()) return;
+    const size_t sl = path.find_last_of('/');
+    if (sl != std::string::npos && sl > 0) {
+        const std::string dir = path.substr(0, sl);
+        const size_t sl2 = dir.find_last_of('/');
+        if (sl2 != std::string::npos && sl2 > 0) (void)mkdir(dir.substr(0, sl2).c_str(), 0755);
+        (void)mkdir(dir.c_str(), 0755);
+    }
+    char line[400];
+    const int len = snprintf(line, sizeof(line), "%s %d %d\n", key.c_str(), idx, graph);
+    const int fd = open(path.c_str(), O_WRONLY | O_APPEND | O_CREAT, 0644);
+    if (fd < 0) return;
+    /* one write of a short line to an O_APPEND file: lines of concurrent
+       writers do not interleave */
+    if (write(fd, line, (size_t)len) != len) { /* a lost cache line only costs a retune */ }
+    close(fd);
+}
+
+/* bv_run_once, choosing the configuration on the first call: the cached
+   choice for this workload if there is one, else each candidate runs once to
+   warm up and twice timed (device time of the factorization), the fastest
+   is kept for the life of the context (and cached), and the call's results
+   come from a final run with it.  Blocks above BV_MAX_MP exclude the fused
+   candidates. */
 static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *out, int *status) {
     if (c->autotune && !c->tuned) {
-        int best = 0;
-        double best_ms = INFINITY;
         const int ncfg = (int)(sizeof(bv_cfgs) / sizeof(bv_cfgs[0]));
+        int kinds_used = 0;
+        for (int e = 0; e < k; ++e) {
+            bv_kparams kp;
+            int kind = 0;
+            if (bv_make_kparams(c, thetas + (size_t)e * nparam, &kp, &kind) == BV_OK)
+                kinds_used |= 1 << kind;
+        }
+        const std::string path = bv_tune_cache_path();
+        const std::string key = bv_tune_key(c, k, kinds_used);
+        int idx = -1, graph = 1;
+        if (bv_tune_cache_get(path, key, &idx, &graph) && idx >= 0 && idx < ncfg &&
+            (c->fused_ok || bv_cfgs[idx].stepped)) {
+            int rc = bv_apply_cfg(c, bv_cfgs[idx]);
+            if (rc) return rc;
+            c->tuned = 1;
+            c->graph_wins = graph != 0;
+            return bv_run_once(c, k, nparam, thetas, out, status);
+        }
+        int best = -1;
+        double best_ms = INFINITY;
         for (int i = 0; i < ncfg; ++i) {
+            if (!c->fused_ok && !bv_cfgs[i].stepped) continue;
             int rc = bv_apply_cfg(c, bv_cfgs[i]);
             if (rc) return rc;
             double ms = INFINITY;
@@ -699,7 +823,7 @@ static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *ou
                 if ((rc = bv_run_once(c, k, nparam, thetas, out, status))) return rc;
                 if (r > 0) ms = std::min(ms, c->last_ms);
             }
-            if (ms < best_ms) { best_ms = ms; best = i; }
+            if (best < 0 || ms < best_ms) { best_ms = ms; best = i; }
         }
         int rc = bv_apply_cfg(c, bv_cfgs[best]);
         if (rc) return rc;
@@ -717,8 +841,22 @@ static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *ou
             }
             c->graph_wins = t[1] < t[0];
         }
+        bv_tune_cache_put(path, key, best, c->graph_wins);
     }
     return bv_run_once(c, k, nparam, thetas, out, status);
+}
+
+extern "C" int bv_factor_candidates(int32_t *cfgs, int max_count) {
+    const int ncfg = (int)(sizeof(bv_cfgs) / sizeof(bv_cfgs[0]));
+    if (cfgs) {
+        for (int i = 0; i < ncfg && i < max_count; ++i) {
+            cfgs[4 * i + 0] = bv_cfgs[i].variant;
+            cfgs[4 * i + 1] = bv_cfgs[i].stepped;
+            cfgs[4 * i + 2] = bv_cfgs[i].nchunks;
+            cfgs[4 * i + 3] = bv_cfgs[i].nstreams;
+        }
+    }
+    return ncfg;
 }
 
 extern "C" int bv_factor_config(bv_ctx *c, int32_t *cfg) {
@@ -883,16 +1021,26 @@ extern "C" int bv_device_selftest(int device) {
  *
  * Each dataset is a bv_ctx (resident in HBM).  Requests (dataset, theta, tag)
  * queue in arrival order.  A lane is one stream with its own workspace,
- * descriptor, slot and result buffers; when a lane is idle, bv_server_poll
- * admits queued requests into it while their estimated workspace bytes fit
- * the lane's budget (at least one request per batch), merges the blocks of
- * all admitted evaluations -- across datasets -- into one launch per
- * covariance kind (blocks sorted by padded size, largest first), and starts
- * it without waiting.  Several lanes run at once, so requests that arrive
- * while a batch is on the GPU are admitted as soon as a lane frees, and their
- * work overlaps on the device.  Every evaluation uses the same kernels and
- * fixed-order reductions as bv_llh, so its result does not depend on the
- * batch it ran in.
+ * descriptor, item, partial-sum, slot and result buffers; when a lane is
+ * idle, bv_server_poll admits queued requests into it while their estimated
+ * workspace bytes fit the lane's budget (at least one request per batch),
+ * merges the blocks of all admitted evaluations -- across datasets -- into
+ * shared launches and starts them without waiting.  Several lanes run at
+ * once, so requests that arrive while a batch is on the GPU are admitted as
+ * soon as a lane frees, and their work overlaps on the device.
+ *
+ * A served result equals the same evaluation run alone by bv_llh, bit for
+ * bit: every evaluation of a dataset runs the factorization form the
+ * dataset's autotuner chose (tuned on its first request if it was not
+ * already).  Blocks are grouped by (covariance kind, form): the fused
+ * kernels (four, two and one wavefronts per block add the same terms in the
+ * same order, so one wavefront is used for merged batches), and each
+ * stepped form with its own work items over the group's blocks (a block's
+ * arithmetic in a stepped form does not depend on the launch or stream its
+ * items run in).  The per-block terms and the per-evaluation sum run in the
+ * fixed orders bv_llh uses.  An evaluation that fills the GPU on its own (a
+ * stepped form and at least BV_SERVER_OWN_BYTES of workspace) runs alone on
+ * the dataset's own launches (streams, chunks, graph), as bv_llh does.
  * ====================================================================== */
 struct bv_request {
     int dataset;
@@ -907,11 +1055,14 @@ struct bv_lane {
     std::vector<bv_request> reqs;            /* the batch in flight          */
     bv_ctx *own = nullptr;                   /* set: one evaluation on its dataset's own
                                                 tuned path, results in h_result */
-    /* capacities (grown while the lane is idle) */
-    int64_t cap_ws = 0, cap_blocks = 0;
-    int cap_evals = 0;
+    /* capacities (sized when datasets are attached, grown while idle) */
+    int64_t cap_ws = 0, cap_blocks = 0, cap_items = 0;
+    int cap_evals = 0, cap_pmax = 0;
     double *d_ws = nullptr;
     bv_block_desc *d_desc = nullptr, *h_desc = nullptr;
+    bv_step_item *d_items = nullptr, *h_items = nullptr;
+    double *d_part_logdet = nullptr, *d_part_dot = nullptr;
+    int32_t *d_part_info = nullptr;
     bv_eval_slot *d_slots = nullptr, *h_slots = nullptr;
     double *d_logdet = nullptr, *d_dot = nullptr, *d_result = nullptr, *h_result = nullptr;
     int32_t *d_info = nullptr;
@@ -921,14 +1072,14 @@ struct bv_server {
     int device = 0;
     double ws_budget = 0.0;                  /* bytes of workspace per lane  */
     int max_evals = 64;                      /* evaluations per batch        */
-    /* kind groups up to this padded size use the one-wavefront factorization
-       kernel: merged batches hold many blocks, and one wavefront per block
-       keeps more of them resident (measured faster at every size tried) */
+    /* fused groups up to this padded size use the one-wavefront kernel:
+       merged batches hold many blocks, and one wavefront per block keeps
+       more of them resident (measured faster at every size tried); the
+       results are those of the four-wavefront kernel */
     int64_t one_wave_max_mp = INT32_MAX;
     /* an evaluation whose dataset's workspace is at least this many bytes,
        and whose autotuned configuration is a stepped form, fills the GPU on
-       its own: it runs alone on its dataset's tuned path (stream, items,
-       configuration) instead of in a merged fused launch */
+       its own: it runs alone on its dataset's own launches */
     double own_min_bytes = 2e8;
     std::vector<bv_ctx *> datasets;
     std::vector<bv_request> queue;           /* FIFO (front = oldest)        */
@@ -940,6 +1091,11 @@ static void bv_lane_free(bv_lane &L) {
     if (L.d_ws) (void)hipFree(L.d_ws);
     if (L.d_desc) (void)hipFree(L.d_desc);
     if (L.h_desc) (void)hipHostFree(L.h_desc);
+    if (L.d_items) (void)hipFree(L.d_items);
+    if (L.h_items) (void)hipHostFree(L.h_items);
+    if (L.d_part_logdet) (void)hipFree(L.d_part_logdet);
+    if (L.d_part_dot) (void)hipFree(L.d_part_dot);
+    if (L.d_part_info) (void)hipFree(L.d_part_info);
     if (L.d_slots) (void)hipFree(L.d_slots);
     if (L.h_slots) (void)hipHostFree(L.h_slots);
     if (L.d_logdet) (void)hipFree(L.d_logdet);
@@ -947,23 +1103,37 @@ static void bv_lane_free(bv_lane &L) {
     if (L.d_info) (void)hipFree(L.d_info);
     if (L.d_result) (void)hipFree(L.d_result);
     if (L.h_result) (void)hipHostFree(L.h_result);
-    L.d_ws = nullptr; L.d_desc = nullptr; L.h_desc = nullptr; L.d_slots = nullptr;
+    L.d_ws = nullptr; L.d_desc = nullptr; L.h_desc = nullptr; L.d_items = nullptr;
+    L.h_items = nullptr; L.d_part_logdet = nullptr; L.d_part_dot = nullptr;
+    L.d_part_info = nullptr; L.d_slots = nullptr;
     L.h_slots = nullptr; L.d_logdet = nullptr; L.d_dot = nullptr; L.d_info = nullptr;
     L.d_result = nullptr; L.h_result = nullptr;
-    L.cap_ws = L.cap_blocks = 0;
-    L.cap_evals = 0;
+    L.cap_ws = L.cap_blocks = L.cap_items = 0;
+    L.cap_evals = L.cap_pmax = 0;
 }
 
-static int bv_lane_reserve(bv_lane &L, int64_t ws, int64_t blocks, int evals) {
-    if (ws <= L.cap_ws && blocks <= L.cap_blocks && evals <= L.cap_evals) return BV_OK;
+static int bv_lane_reserve(bv_lane &L, int64_t ws, int64_t blocks, int evals, int64_t nitems,
+                           int pmax) {
+    if (ws <= L.cap_ws && blocks <= L.cap_blocks && evals <= L.cap_evals &&
+        nitems <= L.cap_items && pmax <= L.cap_pmax)
+        return BV_OK;
     ws = std::max(ws, L.cap_ws);
     blocks = std::max(blocks, L.cap_blocks);
     evals = std::max(evals, L.cap_evals);
+    nitems = std::max<int64_t>(std::max(nitems, L.cap_items), 1);
+    pmax = std::max(std::max(pmax, L.cap_pmax), 1);
     bv_lane_free(L);
-    BV_HIP(hipMalloc((void **)&L.d_ws, sizeof(double) * (size_t)ws));
+    BV_HIP(hipMalloc((void **)&L.d_ws, sizeof(double) * (size_t)std::max<int64_t>(ws, 1)));
     BV_HIP(hipMalloc((void **)&L.d_desc, sizeof(bv_block_desc) * (size_t)blocks));
     BV_HIP(hipHostMalloc((void **)&L.h_desc, sizeof(bv_block_desc) * (size_t)blocks,
                          hipHostMallocDefault));
+    BV_HIP(hipMalloc((void **)&L.d_items, sizeof(bv_step_item) * (size_t)nitems));
+    BV_HIP(hipHostMalloc((void **)&L.h_items, sizeof(bv_step_item) * (size_t)nitems,
+                         hipHostMallocDefault));
+    const size_t np = (size_t)blocks * (size_t)pmax;
+    BV_HIP(hipMalloc((void **)&L.d_part_logdet, sizeof(double) * np));
+    BV_HIP(hipMalloc((void **)&L.d_part_dot, sizeof(double) * np));
+    BV_HIP(hipMalloc((void **)&L.d_part_info, sizeof(int32_t) * np));
     BV_HIP(hipMalloc((void **)&L.d_slots, sizeof(bv_eval_slot) * evals));
     BV_HIP(hipHostMalloc((void **)&L.h_slots, sizeof(bv_eval_slot) * evals, hipHostMallocDefault));
     BV_HIP(hipMalloc((void **)&L.d_logdet, sizeof(double) * (size_t)blocks));
@@ -972,7 +1142,19 @@ static int bv_lane_reserve(bv_lane &L, int64_t ws, int64_t blocks, int evals) {
     BV_HIP(hipMalloc((void **)&L.d_result, sizeof(double) * 4 * evals));
     BV_HIP(hipHostMalloc((void **)&L.h_result, sizeof(double) * 4 * evals, hipHostMallocDefault));
     L.cap_ws = ws; L.cap_blocks = blocks; L.cap_evals = evals;
+    L.cap_items = nitems; L.cap_pmax = pmax;
     return BV_OK;
+}
+
+/* Upper bound of the step items one evaluation of c adds to a merged batch
+   (its blocks' items; the per-launch padding is bounded separately). */
+static int64_t bv_items_per_eval(const bv_ctx *c) {
+    int64_t n = 0;
+    for (const bv_block_desc &d : c->base_desc) {
+        const int64_t T = d.mp / BV_TILE;
+        n += T * T / 4 + T + 1;
+    }
+    return n;
 }
 
 extern "C" int bv_server_create(int device, int nlanes, double ws_bytes_per_lane,
@@ -1002,6 +1184,27 @@ extern "C" int bv_server_create(int device, int nlanes, double ws_bytes_per_lane
     return BV_OK;
 }
 
+/* Capacities a lane needs for a full batch of the attached datasets. */
+static int bv_server_size_lanes(bv_server *s) {
+    int64_t max_b = 0, max_items = 0, ws = (int64_t)(s->ws_budget / 8.0);
+    int pmax = 1;
+    for (const bv_ctx *c : s->datasets) {
+        max_b = std::max<int64_t>(max_b, c->B);
+        max_items = std::max<int64_t>(max_items, bv_items_per_eval(c));
+        ws = std::max<int64_t>(ws, c->ws_per_eval);
+        pmax = std::max(pmax, c->max_mp / BV_NB);
+    }
+    /* padding as in bv_server_launch */
+    const int64_t items = max_items * s->max_evals + (int64_t)(pmax + 1) * 16 * 15;
+    BV_HIP(hipSetDevice(s->device));
+    for (bv_lane &L : s->lanes) {
+        if (L.busy) continue;
+        const int rc = bv_lane_reserve(L, ws, max_b * s->max_evals, s->max_evals, items, pmax);
+        if (rc != BV_OK) return rc;
+    }
+    return BV_OK;
+}
+
 extern "C" int bv_server_attach(bv_server *s, bv_ctx *dataset, int *dataset_id) {
     if (!s || !dataset || !dataset_id) return BV_ERR_ARG;
     if (dataset->device != s->device) return BV_ERR_ARG;
@@ -1009,16 +1212,7 @@ extern "C" int bv_server_attach(bv_server *s, bv_ctx *dataset, int *dataset_id) 
     *dataset_id = (int)s->datasets.size() - 1;
     /* size the idle lanes for a full batch now: growing a lane later frees
        device memory, which waits for every lane's work */
-    int64_t max_b = 0;
-    for (const bv_ctx *c : s->datasets) max_b = std::max<int64_t>(max_b, c->B);
-    const int64_t ws = std::max<int64_t>((int64_t)(s->ws_budget / 8.0), dataset->ws_per_eval);
-    BV_HIP(hipSetDevice(s->device));
-    for (bv_lane &L : s->lanes) {
-        if (L.busy) continue;
-        const int rc = bv_lane_reserve(L, ws, max_b * s->max_evals, s->max_evals);
-        if (rc != BV_OK) return rc;
-    }
-    return BV_OK;
+    return bv_server_size_lanes(s);
 }
 
 extern "C" int bv_server_submit(bv_server *s, int dataset_id, int nparam, const double *theta,
@@ -1043,10 +1237,19 @@ extern "C" int bv_server_submit(bv_server *s, int dataset_id, int nparam, const 
     return BV_OK;
 }
 
-/* Admit queued requests into the idle lane L and launch them. */
 static bool bv_server_own(const bv_server *s, const bv_ctx *c) {
-    return c->stepped > 0 && (c->tuned || !c->autotune) &&
-           8.0 * (double)c->ws_per_eval >= s->own_min_bytes;
+    return c->stepped > 0 && 8.0 * (double)c->ws_per_eval >= s->own_min_bytes;
+}
+
+/* The dataset's factorization configuration is fixed before any of its
+   requests is served: an untuned dataset runs its autotuner now, on the
+   request's parameters (synchronously, on the dataset's own stream and
+   buffers, which no lane uses). */
+static int bv_server_tune(bv_ctx *c, const bv_request &r) {
+    if (!c->autotune || c->tuned) return BV_OK;
+    double v = 0.0;
+    int st = 0;
+    return bv_run(c, 1, bv_nparams(c->kernel), r.theta, &v, &st);
 }
 
 /* One evaluation alone on its dataset's tuned path, enqueued on the dataset's
@@ -1055,7 +1258,7 @@ static bool bv_server_own(const bv_server *s, const bv_ctx *c) {
 static int bv_server_launch_own(bv_server *s, bv_lane &L, const bv_request &r) {
     bv_ctx *c = s->datasets[r.dataset];
     int rc;
-    if ((rc = bv_lane_reserve(L, 0, 0, 1))) return rc;
+    if ((rc = bv_lane_reserve(L, 0, 0, 1, 0, 0))) return rc;
     if ((rc = bv_reserve(c, 1)) || (rc = bv_layout(c, 1))) return rc;
     bv_eval_slot &sl = L.h_slots[0];
     memset(&sl, 0, sizeof(sl));
@@ -1088,37 +1291,55 @@ static int bv_server_launch_own(bv_server *s, bv_lane &L, const bv_request &r) {
     return BV_OK;
 }
 
+/* Admit queued requests into the idle lane L and launch them. */
 static int bv_server_launch(bv_server *s, bv_lane &L) {
     L.reqs.clear();
     L.own = nullptr;
     if (s->qhead < s->queue.size()) {
         const bv_request &r = s->queue[s->qhead];
         bv_ctx *c = s->datasets[r.dataset];
+        int rc = bv_server_tune(c, r);
+        if (rc) return rc;
         if (bv_server_own(s, c)) {
             if (c->srv_busy) return BV_OK; /* its dataset's path is in use: wait */
             ++s->qhead;
             return bv_server_launch_own(s, L, r);
         }
     }
-    int64_t ws = 0, blocks = 0;
+    int64_t ws = 0, blocks = 0, items_bound = 0;
+    int pmax_need = 1;
     while (s->qhead < s->queue.size() && (int)L.reqs.size() < s->max_evals) {
         const bv_request &r = s->queue[s->qhead];
-        const bv_ctx *c = s->datasets[r.dataset];
+        bv_ctx *c = s->datasets[r.dataset];
+        if (!c->tuned && c->autotune) {
+            if (!L.reqs.empty()) break; /* tune it at the head of the next batch */
+            int rc = bv_server_tune(c, r);
+            if (rc) return rc;
+        }
         if (bv_server_own(s, c)) break; /* runs alone on its own path */
         const double bytes = 8.0 * (double)(ws + c->ws_per_eval);
         if (!L.reqs.empty() && bytes > s->ws_budget) break; /* admission by workspace bytes */
         ws += c->ws_per_eval;
         blocks += c->B;
+        items_bound += bv_items_per_eval(c);
+        pmax_need = std::max(pmax_need, c->max_mp / BV_NB);
         L.reqs.push_back(r);
         ++s->qhead;
     }
     const int k = (int)L.reqs.size();
     if (k == 0) return BV_OK;
     int rc;
-    if ((rc = bv_lane_reserve(L, ws, blocks, k))) return rc;
-    /* slots, then descriptors grouped by kind, each group sorted by padded size */
-    int64_t ws_off = 0, blk_off = 0;
-    int kind_of[8] = {0};
+    /* padding: at most 2 * 8 empty items per launch, at most 5 kinds x 3
+       stepped forms of at most pmax_need + 1 launches each */
+    items_bound += (int64_t)(pmax_need + 1) * 16 * 15;
+    if ((rc = bv_lane_reserve(L, ws, blocks, k, items_bound, pmax_need))) return rc;
+    /* groups: (kind, form), form = the dataset's factorization (0 fused,
+       1..3 stepped forms) */
+    const int NG = 8 * 4;
+    int64_t gcount[NG] = {0};
+    int gpmax[NG] = {0};
+    std::vector<int> form_of(k);
+    int64_t blk_off = 0;
     for (int e = 0; e < k; ++e) {
         const bv_request &r = L.reqs[e];
         const bv_ctx *c = s->datasets[r.dataset];
@@ -1135,47 +1356,89 @@ static int bv_server_launch(bv_server *s, bv_lane &L) {
         sl.info = L.d_info + blk_off;
         sl.result = L.d_result + 4 * e;
         sl.log2pi = log(2 * 3.141592653589793);
-        kind_of[sl.kind] += c->B;
+        form_of[e] = c->stepped;
+        const int g = sl.kind * 4 + c->stepped;
+        gcount[g] += c->B;
+        gpmax[g] = std::max(gpmax[g], c->max_mp / BV_NB);
         blk_off += c->B;
     }
-    /* descriptor ranges per kind */
-    int64_t start[8], fill[8], kmax_mp[8];
+    /* descriptors grouped, each group sorted by padded size (largest first) */
+    int64_t start[NG], fill[NG], gmax_mp[NG];
     int64_t acc = 0;
-    for (int kd = 0; kd < 8; ++kd) { start[kd] = acc; fill[kd] = acc; acc += kind_of[kd]; kmax_mp[kd] = 0; }
-    ws_off = 0;
+    for (int g = 0; g < NG; ++g) { start[g] = acc; fill[g] = acc; acc += gcount[g]; gmax_mp[g] = 0; }
+    int64_t ws_off = 0;
     for (int e = 0; e < k; ++e) {
         const bv_ctx *c = s->datasets[L.reqs[e].dataset];
-        const int kd = L.h_slots[e].kind;
+        const int g = L.h_slots[e].kind * 4 + form_of[e];
         for (const bv_block_desc &d0 : c->base_desc) {
             bv_block_desc d = d0;
             d.eval = e;
             d.ws_off = d0.ws_off + ws_off;
-            L.h_desc[fill[kd]++] = d;
-            kmax_mp[kd] = std::max<int64_t>(kmax_mp[kd], d0.mp);
+            L.h_desc[fill[g]++] = d;
+            gmax_mp[g] = std::max<int64_t>(gmax_mp[g], d0.mp);
         }
         ws_off += c->ws_per_eval;
     }
-    for (int kd = 0; kd < 8; ++kd)
-        std::stable_sort(L.h_desc + start[kd], L.h_desc + fill[kd],
+    for (int g = 0; g < NG; ++g)
+        std::stable_sort(L.h_desc + start[g], L.h_desc + fill[g],
                          [](const bv_block_desc &a, const bv_block_desc &b) { return a.mp > b.mp; });
+    /* step items of the stepped groups, one list per group, one chunk */
+    std::vector<bv_step_item> items;
+    std::vector<int64_t> goff[NG];
+    int64_t gitem0[NG] = {0};
+    int64_t nitems = 0;
+    for (int g = 0; g < NG; ++g) {
+        if (!gcount[g] || g % 4 == 0) continue;
+        bv_build_items(L.h_desc + start[g], (size_t)gcount[g], g % 4, 1, false, 1, 1, gpmax[g],
+                       items, goff[g]);
+        if (nitems + (int64_t)items.size() > L.cap_items) return BV_ERR_BUSY; /* not sized */
+        memcpy(L.h_items + nitems, items.data(), sizeof(bv_step_item) * items.size());
+        gitem0[g] = nitems;
+        nitems += (int64_t)items.size();
+    }
     BV_HIP(hipSetDevice(s->device));
     BV_HIP(hipMemcpyAsync(L.d_slots, L.h_slots, sizeof(bv_eval_slot) * k, hipMemcpyHostToDevice,
                           L.stream));
     BV_HIP(hipMemcpyAsync(L.d_desc, L.h_desc, sizeof(bv_block_desc) * (size_t)blocks,
                           hipMemcpyHostToDevice, L.stream));
-    for (int kd = 0; kd < 8; ++kd) {
-        const unsigned nblk = (unsigned)kind_of[kd];
+    if (nitems > 0)
+        BV_HIP(hipMemcpyAsync(L.d_items, L.h_items, sizeof(bv_step_item) * (size_t)nitems,
+                              hipMemcpyHostToDevice, L.stream));
+    const int pst = L.cap_pmax; /* partial-sum stride */
+    for (int g = 0; g < NG; ++g) {
+        const unsigned nblk = (unsigned)gcount[g];
         if (!nblk) continue;
-        const bv_block_desc *dd = L.d_desc + start[kd];
-        const size_t lds = sizeof(double) * 4 * (size_t)kmax_mp[kd];
+        const int kd = g / 4, form = g % 4;
+        const bv_block_desc *dd = L.d_desc + start[g];
         if (kd == BV_KIND_GENERAL || kd == BV_KIND_SPACETIME) {
-            const unsigned T = (unsigned)(kmax_mp[kd] / BV_TILE);
+            const unsigned T = (unsigned)(gmax_mp[g] / BV_TILE);
             BV_HIP(bv_launch_gen(kd, T * (T + 1) / 2, nblk, L.stream, dd, L.d_slots, L.d_ws));
         }
-        if (kmax_mp[kd] <= s->one_wave_max_mp)
-            BV_HIP(bv_launch_block_1w(kd, nblk, lds, L.stream, dd, L.d_slots, L.d_ws));
-        else
-            BV_HIP(bv_launch_block(kd, nblk, lds, L.stream, dd, L.d_slots, L.d_ws));
+        if (form == 0) {
+            const size_t lds = sizeof(double) * 4 * (size_t)gmax_mp[g];
+            if (gmax_mp[g] <= s->one_wave_max_mp)
+                BV_HIP(bv_launch_block_1w(kd, nblk, lds, L.stream, dd, L.d_slots, L.d_ws));
+            else
+                BV_HIP(bv_launch_block(kd, nblk, lds, L.stream, dd, L.d_slots, L.d_ws));
+            continue;
+        }
+        double *pl = L.d_part_logdet + start[g] * pst, *pd = L.d_part_dot + start[g] * pst;
+        int32_t *pi = L.d_part_info + start[g] * pst;
+        const bv_step_item *it0 = L.d_items + gitem0[g];
+        for (int j = 0; j < gpmax[g]; ++j) {
+            const int64_t o = goff[g][j], cnt = goff[g][j + 1] - o;
+            if (cnt <= 0) continue;
+            if (form == 3)
+                BV_HIP(bv_launch_pair(kd, (unsigned)cnt, L.stream, it0 + o, dd, L.d_slots, L.d_ws,
+                                      pl, pd, pi, pst));
+            else if (form == 2)
+                BV_HIP(bv_launch_step2(kd, (unsigned)cnt, L.stream, it0 + o, dd, L.d_slots,
+                                       L.d_ws, pl, pd, pi, pst));
+            else
+                BV_HIP(bv_launch_step(kd, (unsigned)cnt, L.stream, it0 + o, dd, L.d_slots, L.d_ws,
+                                      pl, pd, pi, pst));
+        }
+        BV_HIP(bv_launch_step_parts((int)nblk, L.stream, dd, L.d_slots, pl, pd, pi, pst));
     }
     BV_HIP(bv_launch_reduce((unsigned)k, L.stream, L.d_slots));
     BV_HIP(hipMemcpyAsync(L.h_result, L.d_result, sizeof(double) * 4 * k, hipMemcpyDeviceToHost,

@@ -517,7 +517,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
     __shared__ double sW[3 * 256];
     __shared__ double sT[16 * 17];
     __shared__ double sCol[16], sRow[16], sPiv[BV_NB];
-    __shared__ double s_logdet, s_dotd, s_wdot[NW];
+    __shared__ double s_logdet, s_dotd;
     __shared__ int s_info;
     __shared__ bv_kparams s_kp; /* read where used: not held in registers */
 
@@ -543,7 +543,6 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
     __syncthreads();
 
     double *ws = ws_base + d.ws_off;
-    double dot_lane = 0.0;
 
     for (int j = 0; j < P; ++j) {
         /* thread indices re-derived per panel: an empty asm hides that they are
@@ -587,22 +586,25 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
             const bv_d4 w00 = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
             const bv_d4 w10 = *(const bv_d4 *)(sW + 1 * 256 + 4 * lane);
             const bv_d4 w11 = *(const bv_d4 *)(sW + 2 * 256 + 4 * lane);
+            double dot_lane = 0.0;
             for (int rt = rt0 + 2 + wave; rt < T; rt += NW)
                 bv_tile_solve(ws, T, j, rt, lane, m, cs0, w00, w10, w11, dot_lane);
+            /* the panel's row-m squares go to s_dotd in panel order, reduced by
+               the one wavefront that solved row m's tile: the same sequence of
+               additions for any number of wavefronts per block */
+            const int rtm = m / BV_TILE;
+            if (rtm >= rt0 + 2 && (rtm - rt0 - 2) % NW == wave) {
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) dot_lane += __shfl_xor(dot_lane, off, 64);
+                if (lane == 0) s_dotd += dot_lane;
+            }
         }
         __syncthreads();
     }
 
-    /* fixed-order reduction of the solved-observation squares */
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) dot_lane += __shfl_xor(dot_lane, off, 64);
-    if ((tid0 & 63) == 0) s_wdot[wave] = dot_lane;
-    __syncthreads();
     if (tid0 == 0) {
-        double dsum = s_dotd;
-        for (int w = 0; w < NW; ++w) dsum += s_wdot[w];
         E.logdet[d.block] = s_logdet;
-        E.dot[d.block] = dsum;
+        E.dot[d.block] = s_dotd;
         E.info[d.block] = s_info;
     }
 }
@@ -648,7 +650,6 @@ bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *_
     bv_wave_sync();
 
     double *ws = ws_base + d.ws_off;
-    double dot_lane = 0.0;
     for (int j = 0; j < P; ++j) {
         int lane = lane0;
         asm volatile("" : "+v"(lane)); /* recompute lane-derived values per panel */
@@ -670,16 +671,20 @@ bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *_
         const bv_d4 w00 = *(const bv_d4 *)(sW + 0 * 256 + 4 * lane);
         const bv_d4 w10 = *(const bv_d4 *)(sW + 1 * 256 + 4 * lane);
         const bv_d4 w11 = *(const bv_d4 *)(sW + 2 * 256 + 4 * lane);
+        double dot_lane = 0.0;
         for (int rt = rt0 + 2; rt < T; ++rt)
             bv_tile_solve(ws, T, j, rt, lane, m, cs0, w00, w10, w11, dot_lane);
+        /* row m's squares of this panel, in panel order (as bv_block_kernel) */
+        if (m / BV_TILE >= rt0 + 2) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) dot_lane += __shfl_xor(dot_lane, off, 64);
+            if (lane == 0) s_dotd += dot_lane;
+        }
         bv_wave_sync();
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) dot_lane += __shfl_xor(dot_lane, off, 64);
-    bv_wave_sync();
     if (lane0 == 0) {
         E.logdet[d.block] = s_logdet;
-        E.dot[d.block] = s_dotd + dot_lane;
+        E.dot[d.block] = s_dotd;
         E.info[d.block] = s_info;
     }
 }
@@ -1934,13 +1939,19 @@ hipError_t bv_launch_block_2w(int kind, unsigned nblk, size_t lds, hipStream_t s
 
 hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,
                          const bv_block_desc *desc, const bv_eval_slot *slots, double *ws) {
-    if (kind == BV_KIND_SPACETIME)
-        hipLaunchKernelGGL(bv_gen_kernel<BV_KIND_SPACETIME>, dim3(max_tiles, nblk), dim3(256), 0,
-                           st, desc, slots, ws);
-    else
-        hipLaunchKernelGGL(bv_gen_kernel<BV_KIND_GENERAL>, dim3(max_tiles, nblk), dim3(256), 0,
-                           st, desc, slots, ws);
-    return hipGetLastError();
+    /* grid.y (descriptors) is limited to 65535: launch in slices */
+    for (unsigned off = 0; off < nblk; off += 65535u) {
+        const unsigned cnt = (nblk - off) < 65535u ? (nblk - off) : 65535u;
+        if (kind == BV_KIND_SPACETIME)
+            hipLaunchKernelGGL(bv_gen_kernel<BV_KIND_SPACETIME>, dim3(max_tiles, cnt), dim3(256),
+                               0, st, desc + off, slots, ws);
+        else
+            hipLaunchKernelGGL(bv_gen_kernel<BV_KIND_GENERAL>, dim3(max_tiles, cnt), dim3(256),
+                               0, st, desc + off, slots, ws);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t bv_launch_step(int kind, unsigned nitems, hipStream_t st,

@@ -37,7 +37,10 @@ extern "C" {
 #define BV_ERR_ARG        -1   /* invalid argument (sizes, pointers, theta)       */
 #define BV_ERR_HIP        -2   /* a HIP runtime call failed                       */
 #define BV_ERR_NO_DEVICE  -3   /* no usable gfx950 device                         */
-#define BV_ERR_UNSUPPORTED -4  /* kernel / metric combination not implemented     */
+#define BV_ERR_UNSUPPORTED -4  /* kernel / metric combination not implemented, or a
+                                  fused factorization forced (BV_FACTOR_KERNEL or
+                                  BV_PATH) on blocks of padded augmented size above
+                                  1024 (the other forms take any size)            */
 #define BV_ERR_BUSY       -5   /* serving queue full / workspace exhausted        */
 
 /* ---- covariance kernels (ids of include/vecchia_helper_data.h:482-521) --- */
@@ -121,8 +124,18 @@ double bv_last_kernel_ms(bv_ctx *ctx);
  * evaluation of a context times the candidate configurations on its own data
  * and keeps the fastest, then times graph replay against direct launches for
  * single evaluations (unless BV_FACTOR_KERNEL, BV_PATH or BV_CHUNKS fix the
- * configuration, or BV_AUTOTUNE=0); before that, returns BV_ERR_BUSY. */
+ * configuration, or BV_AUTOTUNE=0); before that, returns BV_ERR_BUSY.  The
+ * choice is cached per device and workload in a text file (BV_TUNE_CACHE,
+ * default $HOME/.cache/blockvecchia_amd/autotune.txt; BV_TUNE_CACHE=0 turns
+ * it off), so later contexts on the machine take it without timing.  Blocks
+ * of padded augmented size above 1024 rule out the fused kernels (cfg[1] = 0);
+ * without autotuning such a dataset uses the lookahead form. */
 int bv_factor_config(bv_ctx *ctx, int32_t *cfg);
+
+/* The autotuner's candidate configurations: writes up to max_count rows of
+ * 4 entries (cfg[0..3] of bv_factor_config) and returns the candidate count.
+ * No counterpart in the reference; for tests that pin every candidate. */
+int bv_factor_candidates(int32_t *cfgs, int max_count);
 
 /* Single evaluations (bv_llh, bv_llh_many with k = 1) may replay a HIP graph
  * of their kernel launches, captured on the first such call after the
@@ -147,14 +160,19 @@ void bv_destroy(bv_ctx *ctx);
  * HBM); requests queue in arrival order; bv_server_poll admits them into idle
  * lanes (one stream each) while their workspace bytes fit the lane budget,
  * merges their blocks into shared launches, and returns finished results.
- * Merged launches factor with one wavefront per block (BV_SERVER_1W_MAXMP=M
- * in the environment at creation limits that to kind groups of padded size
- * <= M; larger ones use four wavefronts).  A request whose dataset is tuned
- * to a stepped form and needs at least BV_SERVER_OWN_BYTES (default 2e8) of
- * workspace runs alone on the dataset's own tuned path and stream, one at a
- * time per dataset; it counts as the context's latest evaluation for
- * bv_block_terms.  Calls on one server are not thread-safe; a merged
- * result does not depend on the batch it ran in. */
+ * Every evaluation runs the factorization form its dataset's autotuner chose
+ * (a dataset not yet tuned is tuned synchronously on its first request), so
+ * a served result equals bv_llh on the same context and parameters bit for
+ * bit, whatever batch it ran in.  Merged blocks are grouped by covariance
+ * kind and form: fused-form blocks share launches with one wavefront per
+ * block (BV_SERVER_1W_MAXMP=M in the environment at creation limits that to
+ * groups of padded size <= M; larger ones use four wavefronts, with the same
+ * results), each stepped form gets its own step launches over its blocks.
+ * A request whose dataset is tuned to a stepped form and needs at least
+ * BV_SERVER_OWN_BYTES (default 2e8) of workspace runs alone on the
+ * dataset's own launches and stream, one at a time per dataset; it counts as
+ * the context's latest evaluation for bv_block_terms.  Calls on one server
+ * are not thread-safe. */
 typedef struct bv_server bv_server;
 
 /* nlanes concurrent batches (1..16), each with ws_bytes_per_lane of

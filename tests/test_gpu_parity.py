@@ -324,13 +324,98 @@ FACTOR_CONFIGS = {
 }
 
 
-@pytest.mark.parametrize("name", sorted(FACTOR_CONFIGS))
-def test_factor_configs_match_oracle(c1_full, name, monkeypatch):
-    """Every factorization configuration the autotuner can choose, fixed by
-    environment (the fused kernel with four, one or two wavefronts per block;
-    the stepped factorization with and without lookahead and by pairs of
-    panels, whole or chunked over streams) against the
+def _candidate_env(cfg):
+    """Environment that fixes one autotuner candidate (variant, stepped,
+    chunks, streams)."""
+    variant, stepped, chunks, streams = cfg
+    env = {"BV_PATH": ["fused", "stepped", "stepped2", "paired"][stepped],
+           "BV_CHUNKS": str(chunks), "BV_CHUNK_STREAMS": str(streams)}
+    if stepped == 0:
+        env["BV_FACTOR_KERNEL"] = ["4w", "1w", "2w"][variant]
+    return env
+
+
+CANDIDATES = bv.factor_candidates()
+
+
+@pytest.mark.parametrize("cfg", CANDIDATES, ids=["v%d_s%d_c%d_st%d" % c for c in CANDIDATES])
+def test_every_candidate_matches_oracle(c1_full, cfg, monkeypatch):
+    """Every configuration in the autotuner's candidate list (read from the
+    library, so the list cannot drift), fixed by environment, against the
     oracle: the n=2000 workload, a general-nu case and a large-block case."""
+    for k, v in _candidate_env(cfg).items():
+        monkeypatch.setenv(k, v)
+    name = "candidate %s" % (cfg,)
+    _say("case: %s n=2000" % name)
+    x, y, obs = c1_full
+    _check(bv.partition(x, y, obs, 100, 60, seed=0), [1.5, 0.1, 0.5])
+    x, y = datagen.uniform_locations(600, 0)
+    obs = datagen.sample_dense(x, y, 1.0, 0.1, 0.5, seed=0)
+    for bc, cs, th in [(30, 40, [1.0, 0.08, 0.8]), (12, 250, [1.0, 0.1, 0.5])]:
+        _say("case: %s n=600 bc=%d cs=%d theta=%s" % (name, bc, cs, th))
+        _check(bv.partition(x, y, obs, bc, cs, seed=0), th)
+
+
+def test_fused_variants_bit_identical(c1_small, monkeypatch):
+    """The fused kernel with four, two and one wavefronts per block adds the
+    same terms in the same order (row m's squares in panel order), so the
+    server may merge fused-form blocks into one-wavefront launches: equal bit
+    for bit, on well- and ill-conditioned parameters and general nu."""
+    x, y, obs = c1_small
+    b = bv.partition(x, y, obs, 30, 100, seed=0)
+    thetas = [[1.5, 0.1, 0.5], [1.0, 0.1, 2.5], [1.0, 0.08, 0.8], [0.7, 0.3, 1.5]]
+    res = {}
+    for v in ("4w", "2w", "1w"):
+        monkeypatch.setenv("BV_FACTOR_KERNEL", v)
+        eng = bv.Engine(b)
+        res[v] = [eng.llh(t) for t in thetas]
+        res[v + "terms"] = eng.block_terms()[:2]
+        eng.close()
+    _say("case: fused variants %s" % res["4w"])
+    assert res["4w"] == res["2w"] == res["1w"]
+    for v in ("2w", "1w"):
+        np.testing.assert_array_equal(res[v + "terms"][0], res["4wterms"][0])
+        np.testing.assert_array_equal(res[v + "terms"][1], res["4wterms"][1])
+
+
+def test_blocks_above_fused_limit(monkeypatch):
+    """Blocks of padded augmented size above 1024 (two k-means blocks of
+    ~1,200 points, no conditioning): the fused kernels refuse them when
+    forced, the autotuner keeps to the stepped forms, and every stepped form
+    matches the oracle."""
+    _say("case: blocks above the fused limit (n=2400, bc=2, cs=0)")
+    x, y = datagen.uniform_locations(2400, 21)
+    obs = np.random.default_rng(21).standard_normal(2400)
+    b = bv.partition(x, y, obs, 2, 0, seed=0)
+    assert int(b.batch_num.max()) + 1 > 1024
+    monkeypatch.setenv("BV_PATH", "fused")
+    with pytest.raises(RuntimeError):
+        bv.Engine(b)
+    monkeypatch.delenv("BV_PATH")
+    th = [1.0, 0.05, 0.5]
+    eng = bv.Engine(b)
+    try:
+        v = eng.llh(th)
+        cfg = eng.factor_config()
+    finally:
+        eng.close()
+    ref = oracle.llh(b, th)
+    _say("  autotuned %s gpu %.17g oracle %.17g rel %.3e" % (cfg, v, ref, rel(v, ref)))
+    assert cfg[1] != 0 and rel(v, ref) <= LLH_RTOL
+    for form in ("stepped", "stepped2", "paired"):
+        monkeypatch.setenv("BV_PATH", form)
+        monkeypatch.setenv("BV_CHUNKS", "1")
+        monkeypatch.setenv("BV_CHUNK_STREAMS", "1")
+        _say("  form %s" % form)
+        _check(b, th)
+
+
+@pytest.mark.parametrize("name", ["stepped_3chunks_2streams", "stepped2_3chunks_2streams",
+                                  "paired_3chunks_2streams", "stepped_2chunks_skewed",
+                                  "paired_3chunks_skewed"])
+def test_factor_configs_match_oracle(c1_full, name, monkeypatch):
+    """Named configurations (fixed by environment), including ones outside
+    the candidate list: more chunks than streams."""
     for k, v in FACTOR_CONFIGS[name].items():
         monkeypatch.setenv(k, v)
     _say("case: %s n=2000" % name)
