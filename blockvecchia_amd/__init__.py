@@ -28,6 +28,7 @@ BV_ERR_HIP = -2
 BV_ERR_NO_DEVICE = -3
 BV_ERR_UNSUPPORTED = -4
 BV_ERR_BUSY = -5
+BV_ERR_SCRATCH = -6
 
 KERNELS = {
     "univariate_matern_stationary_no_nugget": 1,
@@ -82,7 +83,7 @@ EXPORTED = [
     "bv_partition", "bv_blocks_free", "bv_device_selftest",
     "bv_server_create", "bv_server_attach", "bv_server_submit", "bv_server_poll",
     "bv_server_pending", "bv_server_destroy", "bv_factor_config", "bv_set_graph",
-    "bv_partition_gpu", "bv_factor_candidates",
+    "bv_partition_gpu", "bv_factor_candidates", "bv_llh_noalloc",
 ]
 
 _lib = None
@@ -135,6 +136,9 @@ def load_library(path=None):
     lib.bv_set_graph.restype = ctypes.c_int
     lib.bv_factor_candidates.argtypes = [_ip, ctypes.c_int]
     lib.bv_factor_candidates.restype = ctypes.c_int
+    lib.bv_llh_noalloc.argtypes = [ctypes.POINTER(BvProblem), ctypes.c_uint, _dp, _dp,
+                                   ctypes.POINTER(ctypes.c_int)]
+    lib.bv_llh_noalloc.restype = ctypes.c_int
     lib.bv_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_void_p)]
     lib.bv_server_create.restype = ctypes.c_int

@@ -18,6 +18,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1508,6 +1509,168 @@ extern "C" void bv_server_destroy(bv_server *s) {
         if (L.stream) (void)hipStreamDestroy(L.stream);
     }
     delete s;
+}
+
+/* ---- evaluation without allocation --------------------------------------
+   bv_llh_noalloc: one evaluation of a dataset that the caller keeps in device
+   memory, with every buffer the kernels need beyond the inputs and the
+   result (launch descriptors, the evaluation slot, block sizes, per-block
+   terms, the factor workspace) carved out of `scratch`, a file-scope device
+   array of BV_NOALLOC_SCRATCH_BYTES.  Each device holds its own copy of the
+   array, at its own address, from the library's load on that device until
+   the process ends; the host looks that address up once per device
+   (hipGetSymbolAddress, under a lock) and passes it to the kernels.  Every
+   call on a device uses the same array: calls that overlap on one device
+   race.  Nothing in it needs to start at zero: the kernels write every tile,
+   term and result before reading it. */
+static __device__ __align__(256) unsigned char scratch[BV_NOALLOC_SCRATCH_BYTES];
+
+#define BV_MAX_DEVICES 64
+static unsigned char *bv_scratch_dev[BV_MAX_DEVICES];
+static std::mutex bv_scratch_mu;
+
+static int bv_scratch_ptr(int device, unsigned char **out) {
+    std::lock_guard<std::mutex> lk(bv_scratch_mu);
+    if (device < 0 || device >= BV_MAX_DEVICES) return BV_ERR_ARG;
+    if (!bv_scratch_dev[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "libbv: hipGetSymbolAddress(scratch) failed on device %d: %s\n",
+                    device, hipGetErrorString(e));
+            return BV_ERR_HIP;
+        }
+        bv_scratch_dev[device] = (unsigned char *)ptr;
+    }
+    *out = bv_scratch_dev[device];
+    return BV_OK;
+}
+
+static size_t bv_align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+extern "C" int bv_llh_noalloc(const bv_problem *p, unsigned n, const double *theta, double *llh,
+                              int *first_bad_block) {
+    if (!p || !theta || !llh) return BV_ERR_ARG;
+    if (first_bad_block) *first_bad_block = -1;
+    *llh = NAN;
+    if (p->n <= 0 || p->batch_count <= 0 || p->cs < 0 || !p->batch_num || !p->x || !p->y ||
+        !p->obs)
+        return BV_ERR_ARG;
+    if (p->kernel < 1 || p->kernel > 4) return BV_ERR_ARG;
+    if ((p->time_flag || p->kernel == BV_KERNEL_SPACETIME) && !p->z) return BV_ERR_ARG;
+    if (p->distance_metric == 1 && p->time_flag) return BV_ERR_UNSUPPORTED;
+    if (p->cs > 0 && p->batch_count > 1 && !p->nbr && !(p->con_x && p->con_y && p->con_obs))
+        return BV_ERR_ARG;
+    if ((int)n != bv_nparams(p->kernel)) return BV_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || p->device < 0 ||
+        p->device >= ndev)
+        return BV_ERR_NO_DEVICE;
+    BV_HIP(hipSetDevice(p->device));
+    unsigned char *base = nullptr;
+    int rc = bv_scratch_ptr(p->device, &base);
+    if (rc) return rc;
+
+    /* covariance parameters, as bv_create + bv_make_kparams do */
+    bv_ctx tmp;
+    tmp.kernel = p->kernel;
+    tmp.metric = p->distance_metric;
+    tmp.time_flag = p->time_flag || p->kernel == BV_KERNEL_SPACETIME;
+    tmp.dist_scale = p->dist_scale > 0 ? p->dist_scale : 1.0;
+    tmp.d_z = tmp.time_flag ? (double *)p->z : nullptr;
+    bv_eval_slot slot;
+    memset(&slot, 0, sizeof(slot));
+    if ((rc = bv_make_kparams(&tmp, theta, &slot.kp, &slot.kind))) return rc;
+
+    /* descriptors (as bv_create), largest padded size first */
+    const int B = p->batch_count;
+    std::vector<bv_block_desc> desc((size_t)B);
+    int64_t off = 0, acc = 0;
+    int max_mp = 0;
+    for (int i = 0; i < B; ++i) {
+        if (p->batch_num[i] <= 0) return BV_ERR_ARG;
+        bv_block_desc &d = desc[i];
+        memset(&d, 0, sizeof(d));
+        d.block = i;
+        d.cs0 = (i == 0 || p->cs == 0) ? 0 : p->cs;
+        d.m = d.cs0 + p->batch_num[i];
+        d.mp = ((d.m + 1 + BV_NB - 1) / BV_NB) * BV_NB;
+        d.obs_off = acc;
+        acc += p->batch_num[i];
+        const int64_t T = d.mp / BV_TILE;
+        d.ws_off = off;
+        off += T * (T + 1) / 2 * 256;
+        max_mp = std::max(max_mp, d.mp);
+    }
+    if (acc != p->n) return BV_ERR_ARG;
+    if (max_mp > BV_MAX_MP) {
+        fprintf(stderr, "libbv: bv_llh_noalloc factors with the fused kernel, which takes padded "
+                        "augmented sizes up to %d (this dataset needs %d)\n", BV_MAX_MP, max_mp);
+        return BV_ERR_UNSUPPORTED;
+    }
+    std::stable_sort(desc.begin(), desc.end(),
+                     [](const bv_block_desc &a, const bv_block_desc &b) { return a.mp > b.mp; });
+
+    /* scratch layout */
+    const size_t o_slot = 0;
+    const size_t o_res = o_slot + bv_align256(sizeof(bv_eval_slot));
+    const size_t o_bn = o_res + 256;
+    const size_t o_desc = o_bn + bv_align256(sizeof(int32_t) * (size_t)B);
+    const size_t o_ld = o_desc + bv_align256(sizeof(bv_block_desc) * (size_t)B);
+    const size_t o_dot = o_ld + bv_align256(sizeof(double) * (size_t)B);
+    const size_t o_info = o_dot + bv_align256(sizeof(double) * (size_t)B);
+    const size_t o_ws = o_info + bv_align256(sizeof(int32_t) * (size_t)B);
+    const size_t need = o_ws + sizeof(double) * (size_t)off;
+    if (need > (size_t)BV_NOALLOC_SCRATCH_BYTES) {
+        fprintf(stderr, "libbv: bv_llh_noalloc needs %zu bytes of scratch for this dataset, more "
+                        "than the %llu it holds\n", need,
+                (unsigned long long)BV_NOALLOC_SCRATCH_BYTES);
+        return BV_ERR_SCRATCH;
+    }
+    bv_eval_slot *d_slot = (bv_eval_slot *)(base + o_slot);
+    double *d_res = (double *)(base + o_res);
+    int32_t *d_bn = (int32_t *)(base + o_bn);
+    bv_block_desc *d_desc = (bv_block_desc *)(base + o_desc);
+    double *d_ws = (double *)(base + o_ws);
+    slot.x = p->x; slot.y = p->y; slot.z = tmp.d_z; slot.obs = p->obs;
+    if (p->cs > 0 && p->nbr) {
+        slot.nbr = p->nbr;
+    } else if (p->cs > 0) {
+        slot.cx = p->con_x; slot.cy = p->con_y; slot.cobs = p->con_obs;
+        slot.cz = tmp.time_flag ? p->con_z : nullptr;
+    }
+    slot.cs = p->cs;
+    slot.nblocks = B;
+    slot.batch_num = d_bn;
+    slot.logdet = (double *)(base + o_ld);
+    slot.dot = (double *)(base + o_dot);
+    slot.info = (int32_t *)(base + o_info);
+    slot.result = d_res;
+    slot.log2pi = log(2 * 3.141592653589793);
+    const hipStream_t st = 0;
+    BV_HIP(hipMemcpyAsync(d_slot, &slot, sizeof(slot), hipMemcpyHostToDevice, st));
+    BV_HIP(hipMemcpyAsync(d_bn, p->batch_num, sizeof(int32_t) * (size_t)B, hipMemcpyHostToDevice,
+                          st));
+    BV_HIP(hipMemcpyAsync(d_desc, desc.data(), sizeof(bv_block_desc) * (size_t)B,
+                          hipMemcpyHostToDevice, st));
+    if (slot.kind == BV_KIND_GENERAL || slot.kind == BV_KIND_SPACETIME) {
+        const unsigned T = (unsigned)(max_mp / BV_TILE);
+        BV_HIP(bv_launch_gen(slot.kind, T * (T + 1) / 2, (unsigned)B, st, d_desc, d_slot, d_ws));
+    }
+    BV_HIP(bv_launch_block(slot.kind, (unsigned)B, sizeof(double) * 4 * (size_t)max_mp, st,
+                           d_desc, d_slot, d_ws));
+    BV_HIP(bv_launch_reduce(1u, st, d_slot));
+    double res[4];
+    BV_HIP(hipMemcpyAsync(res, d_res, sizeof(res), hipMemcpyDeviceToHost, st));
+    BV_HIP(hipStreamSynchronize(st));
+    const int bad = (int)res[1];
+    if (bad >= 0) {
+        if (first_bad_block) *first_bad_block = bad;
+        *llh = -INFINITY;
+        return BV_ERR_NOT_PD;
+    }
+    *llh = res[0];
+    return BV_OK;
 }
 
 /* ---- pre-processing with the k-means assignment step on the device ------ */

@@ -42,6 +42,7 @@ extern "C" {
                                   BV_PATH) on blocks of padded augmented size above
                                   1024 (the other forms take any size)            */
 #define BV_ERR_BUSY       -5   /* serving queue full / workspace exhausted        */
+#define BV_ERR_SCRATCH    -6   /* dataset too large for bv_llh_noalloc's scratch  */
 
 /* ---- covariance kernels (ids of include/vecchia_helper_data.h:482-521) --- */
 #define BV_KERNEL_MATERN            1  /* univariate_matern_stationary_no_nugget   */
@@ -202,6 +203,30 @@ int bv_server_poll(bv_server *s, int wait, int max_out, int64_t *tags, double *l
 int bv_server_pending(bv_server *s);
 
 void bv_server_destroy(bv_server *s);
+
+/* ---- evaluation without device allocation -------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one log-likelihood evaluation, as bv_llh computes it with
+ * the fused factorization (four wavefronts per block; equal bit for bit to
+ * bv_llh on a context fixed to that form), that calls no hipMalloc or
+ * hipFree.  The dataset stays the caller's: in *p, x, y, z, obs and nbr (or
+ * con_x, con_y, con_z, con_obs) are DEVICE pointers on p->device, while
+ * batch_num is a HOST array.  Everything else the kernels need lives in a
+ * fixed device array of BV_NOALLOC_SCRATCH_BYTES per device, held from the
+ * library's load on a device until the process ends.  A dataset whose
+ * descriptors, per-block terms and factor workspace do not fit returns
+ * BV_ERR_SCRATCH: 128 MiB hold about 3,100 blocks at padded size 96 (cs = 60
+ * with 13-point blocks: n ~ 40,000), 840 at 192 (cs = 150), 310 at 320
+ * (cs = 300); the n = 2,000, cs = 60 workload takes 4.2 MB.  Blocks of padded
+ * augmented size above 1024 return BV_ERR_UNSUPPORTED.  Runs on
+ * the device's null stream and returns when the result is on the host.
+ * Calls that overlap on one device share the array and race: serialise them.
+ * Returns BV_OK, BV_ERR_NOT_PD (*llh = -INFINITY, *first_bad_block set) or an
+ * error. */
+#define BV_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int bv_llh_noalloc(const bv_problem *p, unsigned n, const double *theta, double *llh,
+                   int *first_bad_block);
 
 /* Dense Matern block A[i + j*m] = K(l1[i], l2[j]) computed on the device with
  * the engine's covariance code (core_dcmg, src/ckernel.cpp:1057-1093).
