@@ -83,7 +83,7 @@ EXPORTED = [
     "bv_partition", "bv_blocks_free", "bv_device_selftest",
     "bv_server_create", "bv_server_attach", "bv_server_submit", "bv_server_poll",
     "bv_server_pending", "bv_server_destroy", "bv_factor_config", "bv_set_graph",
-    "bv_partition_gpu", "bv_factor_candidates", "bv_llh_noalloc",
+    "bv_partition_gpu", "bv_factor_candidates", "bv_llh_noalloc", "bv_simulate",
 ]
 
 _lib = None
@@ -139,6 +139,9 @@ def load_library(path=None):
     lib.bv_llh_noalloc.argtypes = [ctypes.POINTER(BvProblem), ctypes.c_uint, _dp, _dp,
                                    ctypes.POINTER(ctypes.c_int)]
     lib.bv_llh_noalloc.restype = ctypes.c_int
+    lib.bv_simulate.argtypes = [ctypes.c_void_p, ctypes.c_uint, _dp, _dp, ctypes.c_double, _dp,
+                                ctypes.POINTER(ctypes.c_int)]
+    lib.bv_simulate.restype = ctypes.c_int
     lib.bv_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_void_p)]
     lib.bv_server_create.restype = ctypes.c_int
@@ -306,6 +309,22 @@ class Engine:
         cfg = np.zeros(5, np.int32)
         rc = self._lib.bv_factor_config(self.ctx, cfg.ctypes.data_as(_ip))
         return tuple(int(v) for v in cfg) if rc == BV_OK else None
+
+    def simulate(self, theta, z, nugget=0.0):
+        """A draw from the Block Vecchia model of this dataset's partition at
+        theta (bv_simulate): z standard normals in block order.  Returns
+        (y in block order, status, number of dependency levels)."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        if len(z) != self.blocks.n:
+            raise ValueError("z must have n entries")
+        y = np.empty(self.blocks.n, np.float64)
+        lv = ctypes.c_int(0)
+        rc = self._lib.bv_simulate(self.ctx, len(th), _d(th), _d(z), float(nugget), _d(y),
+                                   ctypes.byref(lv))
+        if rc not in (BV_OK, BV_ERR_NOT_PD):
+            raise RuntimeError("bv_simulate failed: %d" % rc)
+        return y, rc, lv.value
 
     def set_graph(self, enable):
         """Replay single evaluations from a captured HIP graph (default) or

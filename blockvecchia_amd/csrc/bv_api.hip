@@ -54,6 +54,8 @@ hipError_t bv_launch_matern(int m, int n, const double *x1, const double *y1, co
                             const double *x2, const double *y2, const double *z2,
                             const bv_kparams &kp, int kind, double *A);
 hipError_t bv_launch_selftest(const double *a, const double *b, double *d);
+hipError_t bv_launch_sim_finish(unsigned nblk, hipStream_t st, const bv_block_desc *desc,
+                                const bv_eval_slot *slots, const double *ws, const double *z);
 hipError_t bv_launch_kmeans_assign(int64_t n, const double *px, const double *py, const double *pz,
                                    int k, const double *cx, const double *cy, const double *cz,
                                    int32_t *cluster, hipStream_t st);
@@ -1509,6 +1511,142 @@ extern "C" void bv_server_destroy(bv_server *s) {
         if (L.stream) (void)hipStreamDestroy(L.stream);
     }
     delete s;
+}
+
+/* ---- Block Vecchia simulation ------------------------------------------
+   bv_simulate draws y from the Block Vecchia model of the context's
+   partition: block 0 from its own covariance, block i from y_B | y_N(i).
+   Blocks are grouped into dependency levels (a block's level is one more
+   than the highest level of a block holding one of its conditioning
+   points), so that every block of a level conditions on values of earlier
+   levels only.  Per level: the fused factorization in sample mode of the
+   augmented matrix [S_NN, S_NB; S_BN, S_BB] with row m = [y_N, 0] (general
+   nu: generation first), then bv_sim_finish_kernel writes
+   y_B = L_BN (L_NN^-1 y_N) + L_S z_B. */
+extern "C" int bv_simulate(bv_ctx *c, unsigned n, const double *theta, const double *z,
+                           double nugget, double *y, int *levels) {
+    if (!c || !theta || !z || !y || (int)n != bv_nparams(c->kernel) || !(nugget >= 0.0))
+        return BV_ERR_ARG;
+    if (!c->fused_ok) {
+        fprintf(stderr, "libbv: bv_simulate factors with the fused kernel, which takes padded "
+                        "augmented sizes up to %d (this dataset needs %d)\n", BV_MAX_MP,
+                c->max_mp);
+        return BV_ERR_UNSUPPORTED;
+    }
+    BV_HIP(hipSetDevice(c->device));
+    int rc;
+    if ((rc = bv_reserve(c, 1))) return rc;
+    bv_eval_slot sl;
+    memset(&sl, 0, sizeof(sl));
+    if ((rc = bv_make_kparams(c, theta, &sl.kp, &sl.kind))) return rc;
+    sl.kp.nugget = nugget;
+    /* dependency levels */
+    const int B = c->B, cs = c->cs;
+    std::vector<int32_t> level((size_t)B, 0), nbr;
+    if (cs > 0 && B > 1) {
+        if (!c->d_nbr) return BV_ERR_UNSUPPORTED; /* needs neighbour indices */
+        nbr.resize((size_t)B * cs);
+        BV_HIP(hipMemcpy(nbr.data(), c->d_nbr, sizeof(int32_t) * nbr.size(),
+                         hipMemcpyDeviceToHost));
+        std::vector<int32_t> owner((size_t)c->n);
+        for (int i = 0; i < B; ++i)
+            for (int64_t p = c->h_accum[i]; p < c->h_accum[i + 1]; ++p) owner[(size_t)p] = i;
+        for (int i = 1; i < B; ++i) {
+            int lv = 0;
+            for (int k = 0; k < cs; ++k) {
+                const int o = owner[(size_t)nbr[(size_t)i * cs + k]];
+                if (o >= i) {
+                    fprintf(stderr, "libbv: bv_simulate: block %d conditions on block %d, not an "
+                                    "earlier one\n", i, o);
+                    return BV_ERR_ARG;
+                }
+                lv = std::max(lv, level[(size_t)o] + 1);
+            }
+            level[(size_t)i] = lv;
+        }
+    }
+    const int nlev = B ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+    /* descriptors ordered by level, largest padded size first inside one */
+    std::vector<bv_block_desc> all(c->base_desc);
+    std::stable_sort(all.begin(), all.end(), [&](const bv_block_desc &a, const bv_block_desc &b) {
+        return level[(size_t)a.block] < level[(size_t)b.block];
+    });
+    std::vector<int64_t> lstart((size_t)nlev + 1, 0);
+    std::vector<int32_t> lmaxmp((size_t)nlev, 0);
+    for (const bv_block_desc &d : all) {
+        const int lv = level[(size_t)d.block];
+        lstart[(size_t)lv + 1] += 1;
+        lmaxmp[(size_t)lv] = std::max(lmaxmp[(size_t)lv], d.mp);
+    }
+    for (int l = 0; l < nlev; ++l) lstart[(size_t)l + 1] += lstart[(size_t)l];
+    /* device buffers of the draw: output (read back as y_N), z, descriptors */
+    double *d_y = nullptr, *d_z = nullptr;
+    bv_block_desc *d_desc = nullptr;
+    const size_t nb = sizeof(double) * (size_t)c->n;
+    rc = BV_OK;
+    auto done = [&](int code) {
+        c->last_status = code;
+        if (d_y) (void)hipFree(d_y);
+        if (d_z) (void)hipFree(d_z);
+        if (d_desc) (void)hipFree(d_desc);
+        return code;
+    };
+    if (hipMalloc((void **)&d_y, nb) != hipSuccess || hipMalloc((void **)&d_z, nb) != hipSuccess ||
+        hipMalloc((void **)&d_desc, sizeof(bv_block_desc) * all.size()) != hipSuccess)
+        return done(BV_ERR_HIP);
+    if (hipMemsetAsync(d_y, 0, nb, c->stream) != hipSuccess ||
+        hipMemcpyAsync(d_z, z, nb, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(d_desc, all.data(), sizeof(bv_block_desc) * all.size(),
+                       hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return done(BV_ERR_HIP);
+    sl.x = c->d_x; sl.y = c->d_y; sl.z = c->d_z; sl.obs = d_y;
+    sl.nbr = c->d_nbr; sl.cx = c->d_cx; sl.cy = c->d_cy; sl.cz = c->d_cz; sl.cobs = c->d_cobs;
+    sl.cs = c->cs;
+    sl.nblocks = c->B;
+    sl.batch_num = c->d_batch_num;
+    sl.logdet = c->d_logdet;
+    sl.dot = c->d_dot;
+    sl.info = c->d_info;
+    sl.result = c->d_result;
+    sl.log2pi = log(2 * 3.141592653589793);
+    sl.flags = BV_SLOT_SAMPLE;
+    c->h_slots[0] = sl;
+    if (hipMemcpyAsync(c->d_slots, c->h_slots, sizeof(bv_eval_slot), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+        return done(BV_ERR_HIP);
+    for (int l = 0; l < nlev; ++l) {
+        const bv_block_desc *dd = d_desc + lstart[(size_t)l];
+        const unsigned cnt = (unsigned)(lstart[(size_t)l + 1] - lstart[(size_t)l]);
+        if (!cnt) continue;
+        const int mp = lmaxmp[(size_t)l];
+        hipError_t e = hipSuccess;
+        if (sl.kind == BV_KIND_GENERAL || sl.kind == BV_KIND_SPACETIME) {
+            const unsigned T = (unsigned)(mp / BV_TILE);
+            e = bv_launch_gen(sl.kind, T * (T + 1) / 2, cnt, c->stream, dd, c->d_slots, c->d_ws);
+        }
+        if (e == hipSuccess)
+            e = bv_launch_block(sl.kind, cnt, sizeof(double) * 4 * (size_t)mp, c->stream, dd,
+                                c->d_slots, c->d_ws);
+        if (e == hipSuccess) e = bv_launch_sim_finish(cnt, c->stream, dd, c->d_slots, c->d_ws, d_z);
+        if (e != hipSuccess) {
+            fprintf(stderr, "libbv: bv_simulate level %d: %s\n", l, hipGetErrorString(e));
+            return done(BV_ERR_HIP);
+        }
+    }
+    std::vector<int32_t> info((size_t)B);
+    if (hipMemcpyAsync(y, d_y, nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(info.data(), c->d_info, sizeof(int32_t) * (size_t)B,
+                       hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return done(BV_ERR_HIP);
+    if (levels) *levels = nlev;
+    c->last_k = 0; /* the per-block terms are not a likelihood evaluation's */
+    for (int i = 0; i < B; ++i)
+        if (info[(size_t)i] != 0) {
+            c->last_bad = i;
+            return done(BV_ERR_NOT_PD);
+        }
+    return done(BV_OK);
 }
 
 /* ---- evaluation without allocation --------------------------------------

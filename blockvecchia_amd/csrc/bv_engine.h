@@ -71,8 +71,13 @@ struct bv_eval_slot {
     double *result;                     /* [4] llh, first bad block, ... */
     bv_kparams kp;
     int32_t kind;
-    int32_t pad_;
+    int32_t flags;                      /* BV_SLOT_SAMPLE: store the whole factor */
     double log2pi;                      /* log(2*PI) with PI = 3.141592653589793 */
 };
+
+/* bv_eval_slot.flags: the fused factorization also solves the diagonal
+   blocks' rows (L_jj = C_jj W_jj^T), so the workspace holds the complete
+   lower factor, as the simulation needs. */
+#define BV_SLOT_SAMPLE 1
 
 #endif /* BV_ENGINE_H */

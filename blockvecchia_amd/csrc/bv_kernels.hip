@@ -174,6 +174,11 @@ __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T
                     v[g] = bv_aug_entry<KIND>(p, q, m, xp, yp, zp, sx[q], sy[q], sz[q], so[q], kp);
                 }
             }
+            if (rt == rt0 + ct && kp.nugget != 0.0) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    if (4 * g + lq == lr && p < m) v[g] += kp.nugget;
+            }
             *(bv_d4 *)(ws + bv_tile_off(rt, rt0 + ct, T) + 4 * lane) = v;
             /* one tile row at a time: bounds the generation's live registers */
             __builtin_amdgcn_sched_barrier(0);
@@ -589,6 +594,22 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
             double dot_lane = 0.0;
             for (int rt = rt0 + 2 + wave; rt < T; rt += NW)
                 bv_tile_solve(ws, T, j, rt, lane, m, cs0, w00, w10, w11, dot_lane);
+            if ((E.flags & BV_SLOT_SAMPLE) && wave < 2) {
+                /* the diagonal block's rows too: L_jj = C_jj W^T, so the
+                   workspace ends with the complete factor (their squares of
+                   row m were counted by the diagonal factorization) */
+                double unused = 0.0;
+                if (wave == 1) {
+                    bv_tile_solve(ws, T, j, rt0 + 1, lane, m, cs0, w00, w10, w11, unused);
+                } else {
+                    double *t0 = ws + bv_tile_off(rt0, rt0, T) + 4 * lane;
+                    const bv_d4 c0 = *(const bv_d4 *)t0;
+                    bv_d4 x0 = (bv_d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) x0 = bv_mfma(w00[g], c0[g], x0);
+                    *(bv_d4 *)t0 = x0;
+                }
+            }
             /* the panel's row-m squares go to s_dotd in panel order, reduced by
                the one wavefront that solved row m's tile: the same sequence of
                additions for any number of wavefronts per block */
@@ -1749,7 +1770,49 @@ bv_gen_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__rest
     double v;
     if (TAB && p < d.m && q < d.m) v = bv_cov_tab<KIND>(xp, yp, zp, xq, yq, zq, E.kp, s_tab);
     else v = bv_aug_entry<KIND>(p, q, d.m, xp, yp, zp, xq, yq, zq, oq, E.kp);
+    if (p == q && p < d.m) v += E.kp.nugget; /* simulation only (0 otherwise: exact) */
     ws_base[d.ws_off + bv_tile_off(rt, ct, T) + e] = v;
+}
+
+/* Block Vecchia simulation, after the factorization of a level's blocks in
+   sample mode (the workspace holds the complete lower factor L of each
+   block's augmented matrix, whose row m carries the simulated values y_N of
+   the conditioning set): y_B = L_BN w + L_S z_B with w = L_NN^-1 y_N (row m's
+   first cs0 entries) and L_S the block rows' own factor.  One workgroup per
+   block: v = [w; z_B] staged in LDS, then one thread per block row b sums
+   L(cs0 + b, k) v_k over k <= cs0 + b in order.  Writes y_B into the slot's
+   observation array (the simulation's output, read as y_N by later
+   levels). */
+__device__ __forceinline__ double bv_factor_at(const double *ws, int T, int p, int q) {
+    const int rt = p >> 4, ct = q >> 4, r = p & 15, c = q & 15;
+    return ws[bv_tile_off(rt, ct, T) + 4 * (r + 16 * (c & 3)) + (c >> 2)];
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+bv_sim_finish_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
+                     const double *__restrict__ ws_base, const double *__restrict__ z) {
+    __shared__ double sv[BV_MAX_MP];
+    const bv_block_desc d = desc[blockIdx.x];
+    const bv_eval_slot &E = slots[d.eval];
+    const double *ws = ws_base + d.ws_off;
+    const int T = d.mp / BV_TILE, m = d.m, cs0 = d.cs0;
+    for (int k = threadIdx.x; k < m; k += blockDim.x)
+        sv[k] = (k < cs0) ? bv_factor_at(ws, T, m, k) : z[d.obs_off + (k - cs0)];
+    __syncthreads();
+    double *y = (double *)E.obs;
+    for (int b = threadIdx.x; b < m - cs0; b += blockDim.x) {
+        const int p = cs0 + b;
+        double acc = 0.0;
+        for (int k = 0; k <= p; ++k) acc = fma(bv_factor_at(ws, T, p, k), sv[k], acc);
+        y[d.obs_off + b] = acc;
+    }
+}
+
+hipError_t bv_launch_sim_finish(unsigned nblk, hipStream_t st, const bv_block_desc *desc,
+                                const bv_eval_slot *slots, const double *ws, const double *z) {
+    if (nblk == 0) return hipSuccess;
+    hipLaunchKernelGGL(bv_sim_finish_kernel, dim3(nblk), dim3(256), 0, st, desc, slots, ws, z);
+    return hipGetLastError();
 }
 
 /* The k-means assignment step of the host pre-processing (bv_prep.cpp) on

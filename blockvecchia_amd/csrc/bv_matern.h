@@ -56,6 +56,8 @@ struct bv_kparams {
     double dist_scale;
     double inv_scale;   /* 1 / (beta * dist_scale) */
     double beta_t, nu_t, sep, aux;   /* space-time theta[3..6] */
+    double nugget;      /* added to the diagonal by the simulation only (0 in the
+                           likelihood, which has no nugget, as the reference) */
     int32_t metric;     /* 0 Euclidean, 1 great circle */
     int32_t use_z;      /* 3-D distance */
 };

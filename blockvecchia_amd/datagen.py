@@ -7,8 +7,10 @@ build's own generator and are written in the same CSV formats).
   matern_cov(...)                  dense Matern covariance (closed forms for
                                    nu = 1/2, 3/2, 5/2; scipy K_nu otherwise)
   sample_dense(...)                y = L z, L = chol(Sigma) in FP64
-  sample_vecchia(...)              Vecchia sequential simulation
-  sample_block_vecchia(...)        Block Vecchia simulation over a partition
+  sample_vecchia(...)              Vecchia sequential simulation on the GPU
+                                   (bv_simulate, dependency levels)
+  sample_block_vecchia(...)        the same model drawn on the CPU over a
+                                   given partition (scipy, block by block)
 """
 import numpy as np
 
@@ -85,35 +87,44 @@ def sample_dense(x, y, sigma2, beta, nu, seed=0, chunk=4096, device=None):
     return L @ z
 
 
-def sample_vecchia(x, y, sigma2, beta, nu, cs=400, seed=0):
-    """Sequential simulation y_i | y_{N(i)} with N(i) the cs nearest earlier
-    points (in index order); exact in distribution for the Vecchia
-    approximation of the Matern field."""
-    from scipy.spatial import cKDTree
+def sample_vecchia(x, y, sigma2, beta, nu, cs=400, bc=None, seed=0, device=0, z=None):
+    """Vecchia sequential simulation on the GPU (bv_simulate): the points are
+    partitioned as the likelihood partitions them (k-means into bc blocks,
+    random block order, the cs nearest earlier points of each block's
+    centroid as its conditioning set; bc >= n/2 gives one point per block,
+    the classic point-wise Vecchia order), then each block is drawn from its
+    Gaussian conditional given the values already drawn, block by block in
+    dependency levels.  Exact in distribution for that Vecchia model.  A
+    covariance that is not numerically positive definite in FP64 (smooth
+    kernels on tightly clustered points) is drawn again with the smallest
+    diagonal jitter 1e-12 sigma2 times a power of 10 that succeeds, on every
+    block.  Returns the draw in the points' original order; `nugget` and
+    `levels` on the result say which jitter and how many levels were used.
+    z: n standard normals in the partition's block order (default: seeded)."""
+    import blockvecchia_amd as bv
     n = len(x)
-    rng = np.random.default_rng(seed + 1)
-    z = rng.standard_normal(n)
-    out = np.empty(n)
-    pts = np.column_stack([x, y])
-    for i in range(n):
-        if i == 0:
-            out[0] = np.sqrt(sigma2) * z[0]
-            continue
-        k = min(cs, i)
-        if i <= cs:
-            nb = np.arange(i)
+    if bc is None:
+        bc = max(1, n // 20)
+    blocks = bv.partition(x, y, np.zeros(n), bc, cs, seed=0, device=device)
+    if z is None:
+        z = np.random.default_rng(seed + 1).standard_normal(n)
+    eng = bv.Engine(blocks, device=device)
+    try:
+        for k in [None] + list(range(-12, 0)):
+            nug = 0.0 if k is None else sigma2 * 10.0 ** k
+            yb, rc, levels = eng.simulate([sigma2, beta, nu], z, nugget=nug)
+            if rc == bv.BV_OK:
+                break
         else:
-            tree_pts = pts[:i]
-            d = np.sum((tree_pts - pts[i]) ** 2, axis=1)
-            nb = np.argpartition(d, k - 1)[:k]
-        S22 = matern_cov(x[nb], y[nb], x[nb], y[nb], sigma2, beta, nu)
-        s21 = matern_cov(x[nb], y[nb], x[i:i + 1], y[i:i + 1], sigma2, beta, nu)[:, 0]
-        L = np.linalg.cholesky(S22)
-        a = np.linalg.solve(L, s21)
-        w = np.linalg.solve(L, out[nb])
-        mu = a @ w
-        var = sigma2 - a @ a
-        out[i] = mu + np.sqrt(max(var, 0.0)) * z[i]
+            raise RuntimeError("covariance not positive definite even with 0.1 sigma2 jitter")
+    finally:
+        eng.close()
+    out = np.empty(n)
+    out[np.asarray(blocks.perm)] = yb
+    out = out.view(_Sample)
+    out.nugget = nug
+    out.levels = levels
+    out.blocks = blocks
     return out
 
 
@@ -132,7 +143,7 @@ def write_obs_csv(path, obs):
             f.write("%.17g\n" % v)
 
 
-def sample_block_vecchia(blocks, sigma2, beta, nu, seed=0, threads=None, chunk=256):
+def sample_block_vecchia(blocks, sigma2, beta, nu, seed=0, threads=None, chunk=256, z=None):
     """A draw from the Block Vecchia model of a partition (bv.partition's
     Blocks, points in block order): block 0 from its own covariance, block
     i >= 1 from y_B | y_N(i) with N(i) its conditioning set.  Returns the
@@ -141,7 +152,10 @@ def sample_block_vecchia(blocks, sigma2, beta, nu, seed=0, threads=None, chunk=2
     the conditional means run in block order.  A covariance that is not
     numerically positive definite in FP64 (smooth kernels on tightly clustered
     points) is factored with the smallest diagonal jitter, 1e-12 sigma2 times a
-    power of 10, that succeeds; `blocks_jittered` on the result counts them."""
+    power of 10, that succeeds; `blocks_jittered` on the result counts them.
+    z: standard normals in block order (default: seeded).  The CPU
+    restatement of bv_simulate (the same draw for the same z when no block
+    needs jitter)."""
     from concurrent.futures import ThreadPoolExecutor
 
     from scipy.linalg import LinAlgError, solve_triangular
@@ -169,7 +183,8 @@ def sample_block_vecchia(blocks, sigma2, beta, nu, seed=0, threads=None, chunk=2
     acc = np.concatenate([[0], np.cumsum(bn)])
     nbr = np.asarray(blocks.nbr).reshape(-1, cs) if cs > 0 else None
     B = len(bn)
-    eps = np.random.default_rng(seed + 1).standard_normal(len(x))
+    eps = (np.random.default_rng(seed + 1).standard_normal(len(x)) if z is None
+           else np.asarray(z, dtype=np.float64))
     out = np.empty(len(x))
 
     def prep(i):
@@ -195,5 +210,9 @@ def sample_block_vecchia(blocks, sigma2, beta, nu, seed=0, threads=None, chunk=2
 
 
 class _Sample(np.ndarray):
-    """float64 observations with the count of jittered factorizations."""
+    """float64 observations with the count of jittered factorizations (CPU
+    sampler) or the jitter and dependency levels used (GPU sampler)."""
     blocks_jittered = 0
+    nugget = 0.0
+    levels = 0
+    blocks = None

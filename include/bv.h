@@ -204,6 +204,21 @@ int bv_server_pending(bv_server *s);
 
 void bv_server_destroy(bv_server *s);
 
+/* Draw y (n values, block order) from the Block Vecchia model of the
+ * context's partition at theta: block 0 from its own covariance, each block
+ * i >= 1 from y_B | y_N(i), in dependency levels on the device.  z: n
+ * standard normal deviates (block order); nugget >= 0 is added to every
+ * covariance diagonal (the likelihood never adds one; a small value such as
+ * 1e-12 sigma2 keeps smooth kernels on clustered points numerically positive
+ * definite).  *levels (optional) receives the number of levels.  Needs
+ * neighbour indices (not copies) and blocks of padded size <= 1024.  Returns
+ * BV_OK, BV_ERR_NOT_PD (a block's covariance failed: y is incomplete) or an
+ * error.  No counterpart in the reference, which reads pre-simulated data
+ * (src/test_Xvecchia_batch.cpp:182-237); it generates the inputs of the
+ * large configurations. */
+int bv_simulate(bv_ctx *ctx, unsigned n, const double *theta, const double *z, double nugget,
+                double *y, int *levels);
+
 /* ---- evaluation without device allocation -------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
