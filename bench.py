@@ -174,12 +174,31 @@ def _with_obs(blocks, obs):
     return b
 
 
-def run_uneven(args):
+def cpu_info():
+    """The host's CPU model, logical cores and the cores this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": usable}
+
+
+def measure_uneven(steps=10, warmup=2):
     """Uneven batching: n = 200,000 points, 70% uniform and 30% from 20
     Gaussian clusters (sd 0.02), k-means into 10,000 blocks (1 to ~100
     points each), cs = 400; Matern sigma^2 = 1, range 0.05, nu in {3/2, 5/2};
-    observations from the Block Vecchia model of the partition.  One JSON
-    line: evaluations/s and the factorization's FP64 rate per nu."""
+    observations from the Vecchia sequential simulation with cs = 400 over
+    the same partition (GPU, bv_simulate).  Per nu: evaluations/s, latency,
+    the factorization's FP64 rate, and how many blocks are not numerically
+    positive definite (no nugget, as the reference)."""
     import blockvecchia_amd as bv
     from blockvecchia_amd import datagen, flops
 
@@ -188,22 +207,22 @@ def run_uneven(args):
     t0 = time.time()
     blocks = bv.partition(x, y, np.zeros(n), bc, cs, seed=0, device=0)
     bn = blocks.batch_num
-    log("partition (GPU k-means, host kNN): %.1f s, %d blocks, block sizes %d..%d (block 0: %d)"
+    log("uneven: partition (GPU k-means, host kNN) %.1f s, %d blocks, sizes %d..%d (block 0: %d)"
         % (time.time() - t0, blocks.batch_count, bn[1:].min(), bn[1:].max(), bn[0]))
     F = flops.eval_flops(bn, cs)
     out = []
     for nu in (1.5, 2.5):
         theta = [1.0, 0.05, nu]
         t0 = time.time()
-        obs = datagen.sample_block_vecchia(blocks, *theta, seed=0, threads=16)
-        log("nu=%g observations (Block Vecchia simulation): %.1f s, %d jittered factorizations"
-            % (nu, time.time() - t0, obs.blocks_jittered))
+        obs = datagen.sample_partition(blocks, *theta, seed=0)
+        log("uneven nu=%g: observations (GPU Vecchia simulation, cs=400) %.1f s, %d levels, "
+            "nugget %g" % (nu, time.time() - t0, obs.levels, obs.nugget))
         eng = bv.Engine(_with_obs(blocks, obs))
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             eng.llh(theta)
         lat, kms = [], []
         t_start = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             t1 = time.perf_counter()
             v = eng.llh(theta)
             lat.append(time.perf_counter() - t1)
@@ -211,30 +230,42 @@ def run_uneven(args):
         el = time.perf_counter() - t_start
         info = eng.block_terms()[2]
         kern = float(np.mean(kms))
+        achieved = F / (kern * 1e-3) / 1e12
         out.append({
-            "nu": nu, "evals_per_s": args.steps / el, "ms_per_eval": 1e3 * el / args.steps,
+            "nu": nu, "evals_per_s": steps / el, "ms_per_eval": 1e3 * el / steps,
             "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                            "p99": float(np.percentile(lat, 99) * 1e3)},
             "llh": v, "blocks_not_positive_definite": int(np.count_nonzero(info)),
+            "simulation": {"levels": int(obs.levels), "nugget": float(obs.nugget)},
             "factor_config": list(eng.factor_config()),
-            "roofline": {"bound": "mfma", "achieved": F / (kern * 1e-3) / 1e12,
-                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": F / (kern * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-                         "kernel_ms": kern, "flops_per_launch": F},
+            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                         "kernel_ms": kern, "flops_per_eval": F},
         })
-        log("nu=%g: %.3f ms/eval, %.2f TFLOP/s, %d non-PD blocks" % (
-            nu, out[-1]["ms_per_eval"], out[-1]["roofline"]["achieved"],
-            out[-1]["blocks_not_positive_definite"]))
+        log("uneven nu=%g: %.3f ms/eval (kernels %.3f ms), %.2f TFLOP/s (%.1f%% of peak), %d "
+            "non-PD blocks" % (nu, out[-1]["ms_per_eval"], kern, achieved,
+                               100 * achieved / FP64_PEAK_TFLOPS,
+                               out[-1]["blocks_not_positive_definite"]))
         eng.close()
+    return {"workload": "n=200000 (70% uniform, 30% in 20 clusters sd 0.02), k-means bc=10000, "
+                        "cs=400, sigma2=1 range=0.05, Vecchia-simulated observations (cs=400)",
+            "n": n, "bc": bc, "cs": cs, "batch_count": int(blocks.batch_count),
+            "steps": steps, "warmup": warmup, "per_nu": out}
+
+
+def run_uneven(args):
+    """--workload uneven: the uneven-batching configuration alone, one JSON
+    line (value: evaluations/s at nu = 3/2)."""
+    r = measure_uneven(args.steps, args.warmup)
+    out = r["per_nu"]
     print(json.dumps({
         "metric": METRIC, "value": out[0]["evals_per_s"], "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": out[0]["ms_per_eval"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "uneven batching: n=200000 (70% uniform, 30% in 20 clusters sd "
-                               "0.02), k-means bc=10000, cs=400, sigma2=1 range=0.05; value at "
-                               "nu=3/2", "n": n, "bc": bc, "cs": cs,
-                   "batch_count": int(blocks.batch_count), "parallelism": "single-gpu"},
+        "config": {"workload": "uneven batching: " + r["workload"] + "; value at nu=3/2",
+                   "n": r["n"], "bc": r["bc"], "cs": r["cs"], "batch_count": r["batch_count"],
+                   "parallelism": "single-gpu"},
         "per_nu": out}))
 
 
@@ -265,28 +296,25 @@ def run_serving_c5(args):
         "latency_ms": r["latency_ms"], "serving": r}), flush=True)
 
 
-def run_mle(args):
+def measure_mle():
     """Full maximum-likelihood run with general smoothness: n = 100,000
-    uniform, truth (1.0, 0.08, 0.8), observations from the Block Vecchia
-    model of the k-means partition (bc = 5,000, cs = 150); the driver
-    (blockvecchia_amd/bin/test_dvecchia_batch) maximises from 0.1:0.1:0.1
-    with xtol 1e-4.  One JSON line: wall-clock to convergence (lower is
-    better), evaluations, estimates and the final log-likelihood."""
+    uniform, truth (1.0, 0.08, 0.8), observations from the Vecchia
+    sequential simulation with cs = 400 (GPU, its own partition into 5,000
+    blocks); the driver (blockvecchia_amd/bin/test_dvecchia_batch) partitions
+    into bc = 5,000 blocks with cs = 150 and maximises from 0.1:0.1:0.1 with
+    xtol 1e-4.  Wall-clock to convergence, evaluations, estimates and the
+    final log-likelihood."""
     import subprocess
     import tempfile
 
-    import blockvecchia_amd as bv
     from blockvecchia_amd import datagen
 
     n, bc, cs = 100000, 5000, 150
     theta = [1.0, 0.08, 0.8]
     x, y = datagen.uniform_locations(n, 0)
-    blocks = bv.partition(x, y, np.zeros(n), bc, cs, seed=0, device=0)
     t0 = time.time()
-    obs_b = datagen.sample_block_vecchia(blocks, *theta, seed=0, threads=16)
-    obs = np.empty(n)
-    obs[np.asarray(blocks.perm)] = np.asarray(obs_b)
-    log("observations (Block Vecchia simulation): %.1f s" % (time.time() - t0))
+    obs = np.asarray(datagen.sample_vecchia(x, y, *theta, cs=400, bc=bc, seed=0))
+    log("mle: observations (GPU Vecchia simulation, cs=400): %.1f s" % (time.time() - t0))
     driver = os.path.join(ROOT, "blockvecchia_amd", "bin", "test_dvecchia_batch")
     with tempfile.TemporaryDirectory() as tmp:
         datagen.write_xy_csv(os.path.join(tmp, "xy.csv"), x, y)
@@ -295,7 +323,7 @@ def run_mle(args):
                os.path.join(tmp, "obs.csv"), "--num_loc", str(n), "--vecchia_bc", str(bc),
                "--vecchia_cs", str(cs), "--kernel_init", "0.1:0.1:0.1", "--ikernel", "?:?:?",
                "--tol", "4", "--omp_threads", "16", "--seed", "0"]
-        log("driver: " + " ".join(cmd))
+        log("mle driver: " + " ".join(cmd))
         t0 = time.perf_counter()
         r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True, timeout=900)
         wall = time.perf_counter() - t0
@@ -306,16 +334,26 @@ def run_mle(args):
                 open(os.path.join(tmp, "log", logs[0])).read().strip().splitlines()[1].split(",")]
     iters, est, llh, opt_s = int(vals[0]), [float(v) for v in vals[1:4]], float(vals[4]), \
         float(vals[5])
+    log("mle: %d evaluations, %.2f s to convergence, estimate %s, llh %.10g" % (
+        iters, opt_s, est, llh))
+    return {"workload": "MLE: n=100000 uniform, truth (1.0, 0.08, 0.8), Vecchia-simulated "
+                        "observations (cs=400), bc=5000, cs=150, init 0.1:0.1:0.1, xtol 1e-4",
+            "n": n, "bc": bc, "cs": cs, "optimization_s": opt_s, "evaluations": iters,
+            "ms_per_eval": 1e3 * opt_s / iters, "estimate": est, "truth": theta, "llh": llh,
+            "driver_wall_s": wall}
+
+
+def run_mle(args):
+    r = measure_mle()
     print(json.dumps({
-        "metric": "MLE wall-clock to convergence (driver, general nu)", "value": opt_s,
-        "unit": "s", "n_gpus": 1, "steps": iters, "warmup": 0, "ms_per_step": 1e3 * opt_s / iters,
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "MLE: n=100000 uniform, truth (1.0, 0.08, 0.8), Block Vecchia "
-                               "observations, bc=5000, cs=150, init 0.1:0.1:0.1, xtol 1e-4",
-                   "n": n, "bc": bc, "cs": cs, "parallelism": "single-gpu"},
-        "evaluations": iters, "estimate": est, "truth": theta, "llh": llh,
-        "driver_wall_s": wall}))
+        "metric": "MLE wall-clock to convergence (driver, general nu)", "value": r["optimization_s"],
+        "unit": "s", "n_gpus": 1, "steps": r["evaluations"], "warmup": 0,
+        "ms_per_step": r["ms_per_eval"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": r["workload"], "n": r["n"], "bc": r["bc"], "cs": r["cs"],
+                   "parallelism": "single-gpu"},
+        "evaluations": r["evaluations"], "estimate": r["estimate"], "truth": r["truth"],
+        "llh": r["llh"], "driver_wall_s": r["driver_wall_s"]}))
 
 
 def main():
@@ -327,6 +365,10 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving scenarios (many sessions on resident datasets)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other configurations reported under extra keys of the "
+                         "default run (uneven batching, the general-nu MLE, many-dataset "
+                         "serving)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="append a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -418,26 +460,38 @@ def main():
     cpu = None
     rel_err = None
     if not closed and not args.no_cpu_baseline:
-        # the oracle's K_nu is a long-double quadrature: a full evaluation on
-        # the host would take minutes
+        # the host chain covers the closed-form kinds (the oracle's K_nu is a
+        # long-double quadrature: minutes per evaluation)
         log("cpu baseline skipped for general nu")
     if closed and not args.no_cpu_baseline:
         import oracle
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-        oracle.llh(blocks, theta, threads=threads)  # warm-up
-        ts = []
-        ov = None
-        for _ in range(args.cpu_repeats):
-            t0 = time.perf_counter()
-            ov = oracle.llh(blocks, theta, threads=threads)
-            ts.append(time.perf_counter() - t0)
-        med = float(np.median(ts))
+        info = cpu_info()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["usable_cores"]
+        threads = min(threads, info["usable_cores"])
+        # FP64 agreement with the oracle's op-by-op restatement (one evaluation)
+        ov = oracle.llh(blocks, theta, threads=threads)
         rel_err = abs(llh - ov) / abs(ov)
-        cpu = {"value": 1.0 / med, "unit": "evals/s", "cores": threads, "kind": "port",
-               "sample": "the full n=20000 evaluation (all %d blocks), median of %d after one "
-                         "warm-up; CPU restatement of the reference chain (oracle/oracle.c, "
-                         "OpenMP over blocks)" % (blocks.batch_count, args.cpu_repeats)}
-        log("cpu port %.3f s/eval on %d threads, gpu vs cpu rel err %.3e" % (med, threads, rel_err))
+        if oracle.lapack_available():
+            oracle.lapack_llh(blocks, theta, threads=threads)  # warm-up
+            ts = []
+            lv = None
+            for _ in range(args.cpu_repeats):
+                t0 = time.perf_counter()
+                lv = oracle.lapack_llh(blocks, theta, threads=threads)
+                ts.append(time.perf_counter() - t0)
+            med = float(np.median(ts))
+            cpu = {"value": 1.0 / med, "unit": "evals/s", "cores": threads, "kind": "port",
+                   "library": oracle.lapack_library(), "cpu_model": info["cpu_model"],
+                   "nproc": info["nproc"], "usable_cores": info["usable_cores"],
+                   "rel_err_vs_gpu": abs(lv - llh) / abs(llh),
+                   "sample": "the full n=20000 evaluation (all %d blocks), median of %d after "
+                             "one warm-up: the reference's per-block chain (dpotrf, dtrsm, dgemm, "
+                             "dnrm2; oracle/lapack_chain.c) through MKL, OpenMP over blocks "
+                             "with one MKL thread each" % (blocks.batch_count, args.cpu_repeats)}
+            log("cpu baseline (MKL) %.3f s/eval on %d of %d cores (%s); gpu vs oracle rel err "
+                "%.3e" % (med, threads, info["nproc"], info["cpu_model"], rel_err))
+        else:
+            log("cpu baseline: MKL (libmkl_rt.so) not found; not timed")
 
     serving = None
     if not args.no_serving:
@@ -450,7 +504,36 @@ def main():
                  r["latency_ms"]["p50"], r["latency_ms"]["p99"]))
         log("serving scenarios: %.1f s" % (time.perf_counter() - t0))
 
+    extra = {}
+    if not args.no_extra:
+        for key, fn in (("uneven", lambda: measure_uneven(10, 2)), ("mle", measure_mle),
+                        ("serving_c5", lambda: __import__("blockvecchia_amd.serving",
+                                                          fromlist=["x"]).benchmark_c5(log=log))):
+            t0 = time.perf_counter()
+            try:
+                extra[key] = fn()
+            except Exception as e:  # report and keep the main line
+                extra[key] = {"error": "%s: %s" % (type(e).__name__, e)}
+            log("%s: %.1f s" % (key, time.perf_counter() - t0))
+
     traffic = traffic_from_profiles(cfg) if args.nu == 0.5 else None
+    nlaunch = {0: 1, 1: eng_pmax(blocks, cs), 2: eng_pmax(blocks, cs),
+               3: (eng_pmax(blocks, cs) + 1) // 2}[cfg[1] if cfg else 0]
+    if cfg and cfg[1] and cfg[2] > 1 and cfg[3] != 0:
+        nlaunch *= cfg[2]
+    limiter = None
+    if traffic:
+        hbm_rate = traffic / (kern_ms * 1e-3) / 1e9
+        limiter = {
+            "hbm_bytes_per_eval": traffic, "hbm_GBps": hbm_rate,
+            "hbm_frac": hbm_rate / HBM_PEAK_GBS,
+            "reading": "algorithmically MFMA-bound (%.1f GFLOP on 2.3 MB of inputs), measured "
+                       "memory-bound: the factor tiles make %.2f GB of HBM round trips per "
+                       "evaluation at %.0f%% of HBM peak while the matrix cores run at %.0f%%; "
+                       "waves wait on memory results for 42-49%% of their cycles "
+                       "(profiles/factorization_counters_n20000_cs300.md)"
+                       % (F / 1e9, traffic / 1e9, 100 * hbm_rate / HBM_PEAK_GBS,
+                          100 * achieved / FP64_PEAK_TFLOPS)}
     out = {
         "metric": METRIC,
         "value": evals_per_s,
@@ -482,14 +565,17 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
+            "measured_limiter": limiter,
             "kernel": kname,
             "config": cfg,
-            "flops_per_launch": F,
+            "flops_per_eval": F,
+            "launches_per_eval": nlaunch,
             "kernel_ms": kern_ms,
         },
         "cpu_baseline": cpu,
         "serving": serving,
     }
+    out.update(extra)
     eng.close()
     print(json.dumps(out), flush=True)
 

@@ -9,6 +9,7 @@ build's own generator and are written in the same CSV formats).
   sample_dense(...)                y = L z, L = chol(Sigma) in FP64
   sample_vecchia(...)              Vecchia sequential simulation on the GPU
                                    (bv_simulate, dependency levels)
+  sample_partition(...)            the same over a given partition (GPU)
   sample_block_vecchia(...)        the same model drawn on the CPU over a
                                    given partition (scipy, block by block)
 """
@@ -106,9 +107,26 @@ def sample_vecchia(x, y, sigma2, beta, nu, cs=400, bc=None, seed=0, device=0, z=
     if bc is None:
         bc = max(1, n // 20)
     blocks = bv.partition(x, y, np.zeros(n), bc, cs, seed=0, device=device)
+    yb = sample_partition(blocks, sigma2, beta, nu, seed=seed, device=device, z=z)
+    out = np.empty(n)
+    out[np.asarray(blocks.perm)] = yb
+    out = out.view(_Sample)
+    out.nugget = yb.nugget
+    out.levels = yb.levels
+    out.blocks = blocks
+    return out
+
+
+def sample_partition(blocks, sigma2, beta, nu, seed=0, device=0, z=None):
+    """A draw from the Block Vecchia model of a given partition on the GPU
+    (bv_simulate), in block order, with the jitter rule of sample_vecchia.
+    z: standard normals in block order (default: seeded as sample_vecchia)."""
+    import blockvecchia_amd as bv
+    n = blocks.n
     if z is None:
         z = np.random.default_rng(seed + 1).standard_normal(n)
-    eng = bv.Engine(blocks, device=device)
+    eng = bv.Engine(bv.Blocks(blocks.x, blocks.y, np.zeros(n), blocks.batch_num, blocks.nbr,
+                              blocks.cs), device=device)
     try:
         for k in [None] + list(range(-12, 0)):
             nug = 0.0 if k is None else sigma2 * 10.0 ** k
@@ -119,12 +137,9 @@ def sample_vecchia(x, y, sigma2, beta, nu, cs=400, bc=None, seed=0, device=0, z=
             raise RuntimeError("covariance not positive definite even with 0.1 sigma2 jitter")
     finally:
         eng.close()
-    out = np.empty(n)
-    out[np.asarray(blocks.perm)] = yb
-    out = out.view(_Sample)
+    out = np.ascontiguousarray(yb).view(_Sample)
     out.nugget = nug
     out.levels = levels
-    out.blocks = blocks
     return out
 
 

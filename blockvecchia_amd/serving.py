@@ -245,13 +245,10 @@ def build_c5_datasets(count=32, seed=5, log=None):
     """The many-dataset serving configuration: `count` datasets with n drawn
     from {5k, 20k, 50k, 200k} uniform on [0,1]^2, cs from {30, 60, 150,
     300}, n/13 k-means blocks, Matern nu from {1/2, 3/2, 5/2}, sigma^2 = 1.5,
-    range 0.1.  Observations are model draws: the dense FP64 Cholesky (on the
-    GPU) for n <= 20k, the Block Vecchia model of the dataset's own partition
-    (sequential simulation on the host) for n >= 50k.  White noise would make
-    the quadratic forms so ill-conditioned that two orders of the same FP64
-    operations disagree far beyond rounding."""
-    import torch
-    device = "cuda" if torch.cuda.is_available() else None
+    range 0.1.  Observations are draws from the Block Vecchia model of the
+    dataset's own partition (GPU sequential simulation, bv_simulate).  White
+    noise would make the quadratic forms so ill-conditioned that two orders of
+    the same FP64 operations disagree far beyond rounding."""
     rng = np.random.default_rng(seed)
     specs, engines, truths = [], [], []
     for i in range(count):
@@ -259,16 +256,9 @@ def build_c5_datasets(count=32, seed=5, log=None):
         cs = int(rng.choice([30, 60, 150, 300]))
         nu = float(rng.choice([0.5, 1.5, 2.5]))
         x, y = datagen.uniform_locations(n, seed + 100 + i)
-        if n <= 20000:
-            obs = datagen.sample_dense(x, y, 1.5, 0.1, nu, seed=seed + i, device=device)
-            blocks = bv.partition(x, y, obs, max(1, n // 13), cs, seed=0,
-                                  device=0 if device else None)
-        else:
-            blocks = bv.partition(x, y, np.zeros(n), max(1, n // 13), cs, seed=0,
-                                  device=0 if device else None)
-            blocks.obs = np.ascontiguousarray(
-                datagen.sample_block_vecchia(blocks, 1.5, 0.1, nu, seed=seed + i, threads=16),
-                dtype=np.float64)
+        blocks = bv.partition(x, y, np.zeros(n), max(1, n // 13), cs, seed=0, device=0)
+        blocks.obs = np.ascontiguousarray(
+            datagen.sample_partition(blocks, 1.5, 0.1, nu, seed=seed + i), dtype=np.float64)
         engines.append(bv.Engine(blocks))
         truths.append(np.array([1.5, 0.1, nu]))
         specs.append((n, cs, nu))
@@ -305,9 +295,9 @@ def benchmark_c5(count=4096, load=0.6, lanes=4, seed=5, log=None):
     if log:
         log("c5 plain: %.1f s" % t_p)
     worst = max(abs(served[k] - plain[k]) / abs(plain[k]) for k in plain)
-    # per dataset: the served and plain results of one request come from
-    # different kernels (merged one-wavefront launches or the dataset's tuned
-    # path), so they differ by the conditioning of that dataset's likelihood
+    identical = sum(1 for k in plain if served[k] == plain[k])
+    # per dataset: served and plain results of one request run the dataset's
+    # tuned factorization form, so they agree bit for bit
     per_ds = {}
     for k in plain:
         d = reqs[k][1]
@@ -325,6 +315,7 @@ def benchmark_c5(count=4096, load=0.6, lanes=4, seed=5, log=None):
         "plain_latency_ms": {"p50": float(np.percentile(lat_p, 50) * 1e3),
                              "p99": float(np.percentile(lat_p, 99) * 1e3)},
         "max_rel_diff_vs_alone": worst,
+        "results_identical_to_alone": identical,
         "max_rel_diff_by_dataset": [per_ds.get(d, 0.0) for d in range(len(specs))],
         "closed_loop": {"sessions": 32, "requests": nplan,
                         "evals_per_s": capacity, "plain_evals_per_s": nplan / t_pcl,

@@ -94,3 +94,49 @@ def llh(blocks, theta, kernel=1, metric=0, time_flag=False, dist_scale=1.0, thre
 def dense_llh(x, y, obs, theta, kernel=1, metric=0, dist_scale=1.0, z=None):
     return _load().oracle_dense_llh(len(x), _d(x), _d(y), _d(z), _d(obs), _d(theta), kernel,
                                     metric, dist_scale)
+
+
+# ---- the LAPACK (MKL) chain: the CPU baseline ------------------------------
+LAPACK_LIB = os.path.join(HERE, "liblapackchain.so")
+_lap = None
+
+
+def lapack_available():
+    """True when liblapackchain.so loads and finds MKL (libmkl_rt)."""
+    global _lap
+    if _lap is None:
+        if not os.path.exists(LAPACK_LIB):
+            from . import build
+            build.build_oracle()
+        lib = ctypes.CDLL(LAPACK_LIB)
+        lib.lapack_chain_open.argtypes = [ctypes.c_char_p]
+        lib.lapack_chain_open.restype = ctypes.c_int
+        lib.lapack_chain_library.argtypes = []
+        lib.lapack_chain_library.restype = ctypes.c_char_p
+        lib.lapack_chain_llh.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _ip, _dp,
+                                         _dp, _dp, _ip, _dp, ctypes.c_int, _dp, _dp]
+        lib.lapack_chain_llh.restype = ctypes.c_double
+        _lap = lib
+    return _lap.lapack_chain_open(os.environ.get("BV_MKL_PATH", "").encode()) == 0
+
+
+def lapack_library():
+    return _lap.lapack_chain_library().decode() if lapack_available() else None
+
+
+def lapack_llh(blocks, theta, threads=1, per_block=False):
+    """The reference chain through MKL's dpotrf/dtrsm/dgemm/dnrm2 per block
+    (OpenMP over blocks, one MKL thread each); closed-form nu only."""
+    if not lapack_available():
+        raise OSError("MKL (libmkl_rt.so) not found")
+    b = blocks
+    B = len(b.batch_num)
+    bn = np.ascontiguousarray(b.batch_num, dtype=np.int32)
+    nbr = np.ascontiguousarray(b.nbr, dtype=np.int32) if b.cs > 0 else np.zeros(1, np.int32)
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (b.x, b.y, b.obs)]
+    ld, dt = np.empty(B), np.empty(B)
+    v = _lap.lapack_chain_llh(len(b.x), B, b.cs, bn.ctypes.data_as(_ip), _d(keep[0]),
+                              _d(keep[1]), _d(keep[2]), nbr.ctypes.data_as(_ip), _d(th),
+                              int(threads), ld.ctypes.data_as(_dp), dt.ctypes.data_as(_dp))
+    return (v, ld, dt) if per_block else v

@@ -249,3 +249,22 @@ def test_not_positive_definite_reports_info():
     b = _blocks_from(x, y, np.ones(4), [4], np.zeros(0), 0)
     v, ld, dt, info = oracle.llh(b, [1.0, 0.1, 0.5], per_block=True)
     assert info[0] == 2 and math.isnan(v)
+
+
+def test_lapack_chain_equals_oracle(c1_small):
+    """The CPU baseline (the reference chain through MKL per block) gives the
+    oracle's log-likelihood and per-block terms."""
+    import blockvecchia_amd as bv
+    if not oracle.lapack_available():
+        pytest.skip("MKL (libmkl_rt.so) not available")
+    x, y, obs = c1_small
+    b = bv.partition(x, y, obs, 30, 40, seed=0)
+    # well-conditioned parameters (the data are a nu = 1/2 draw): a smooth
+    # misspecified kernel amplifies rounding far past 1e-11
+    for th in ([1.5, 0.1, 0.5], [0.9, 0.05, 1.5], [1.1, 0.03, 2.5]):
+        v, ld, dt = oracle.lapack_llh(b, th, threads=4, per_block=True)
+        ov, old, odt, _ = oracle.llh(b, th, per_block=True)
+        assert abs(v - ov) <= 1e-11 * abs(ov), (th, v, ov)
+        np.testing.assert_allclose(ld, old, rtol=1e-10, atol=1e-10)
+        np.testing.assert_allclose(dt, odt, rtol=1e-10, atol=1e-10)
+    assert oracle.lapack_library()
