@@ -59,6 +59,9 @@ hipError_t bv_launch_sim_finish(unsigned nblk, hipStream_t st, const bv_block_de
 hipError_t bv_launch_kmeans_assign(int64_t n, const double *px, const double *py, const double *pz,
                                    int k, const double *cx, const double *cy, const double *cz,
                                    int32_t *cluster, hipStream_t st);
+hipError_t bv_launch_knn(const double *x, const double *y, const double *z, const int64_t *accum,
+                         const double *bcx, const double *bcy, const double *bcz, int64_t B,
+                         int k, int time_flag, int32_t *nbr, int32_t *overflow, hipStream_t st);
 
 #define BV_HIP(call)                                                                    \
     do {                                                                                \
@@ -1876,6 +1879,50 @@ static int bv_km_setup(bv_km_dev &D, int64_t n, const double *x, const double *y
     return BV_OK;
 }
 
+/* The conditioning sets on the device (bv_knn_kernel), for bv_partition_run. */
+static int bv_knn_dev(void *user, int64_t n, const double *x, const double *y, const double *z,
+                      int64_t B, const int64_t *accum, const double *cx, const double *cy,
+                      const double *cz, int32_t cs, int32_t time_flag, int32_t *nbr,
+                      int32_t *redo) {
+    bv_km_dev &D = *(bv_km_dev *)user;
+    const size_t nb = sizeof(double) * (size_t)n, bb = sizeof(double) * (size_t)B;
+    double *dx = nullptr, *dy = nullptr, *dz = nullptr, *dcx = nullptr, *dcy = nullptr,
+           *dcz = nullptr;
+    int64_t *dacc = nullptr;
+    int32_t *dnbr = nullptr, *dredo = nullptr;
+    int rc = BV_ERR_HIP;
+    const size_t nn = (size_t)B * (size_t)cs;
+    if (hipMalloc((void **)&dx, nb) == hipSuccess && hipMalloc((void **)&dy, nb) == hipSuccess &&
+        (!time_flag || hipMalloc((void **)&dz, nb) == hipSuccess) &&
+        hipMalloc((void **)&dcx, bb) == hipSuccess && hipMalloc((void **)&dcy, bb) == hipSuccess &&
+        hipMalloc((void **)&dcz, bb) == hipSuccess &&
+        hipMalloc((void **)&dacc, sizeof(int64_t) * (size_t)(B + 1)) == hipSuccess &&
+        hipMalloc((void **)&dnbr, sizeof(int32_t) * nn) == hipSuccess &&
+        hipMalloc((void **)&dredo, sizeof(int32_t) * (size_t)B) == hipSuccess &&
+        hipMemcpyAsync(dx, x, nb, hipMemcpyHostToDevice, D.st) == hipSuccess &&
+        hipMemcpyAsync(dy, y, nb, hipMemcpyHostToDevice, D.st) == hipSuccess &&
+        (!time_flag || hipMemcpyAsync(dz, z, nb, hipMemcpyHostToDevice, D.st) == hipSuccess) &&
+        hipMemcpyAsync(dcx, cx, bb, hipMemcpyHostToDevice, D.st) == hipSuccess &&
+        hipMemcpyAsync(dcy, cy, bb, hipMemcpyHostToDevice, D.st) == hipSuccess &&
+        hipMemcpyAsync(dcz, cz, bb, hipMemcpyHostToDevice, D.st) == hipSuccess &&
+        hipMemcpyAsync(dacc, accum, sizeof(int64_t) * (size_t)(B + 1), hipMemcpyHostToDevice,
+                       D.st) == hipSuccess &&
+        hipMemsetAsync(dredo, 0, sizeof(int32_t) * (size_t)B, D.st) == hipSuccess &&
+        bv_launch_knn(dx, dy, dz, dacc, dcx, dcy, dcz, B, cs, time_flag, dnbr, dredo, D.st) ==
+            hipSuccess &&
+        /* row 0 is the host's (0..cs-1); rows 1..B-1 come back */
+        hipMemcpyAsync(nbr + cs, dnbr + cs, sizeof(int32_t) * (nn - (size_t)cs),
+                       hipMemcpyDeviceToHost, D.st) == hipSuccess &&
+        hipMemcpyAsync(redo, dredo, sizeof(int32_t) * (size_t)B, hipMemcpyDeviceToHost, D.st) ==
+            hipSuccess &&
+        hipStreamSynchronize(D.st) == hipSuccess)
+        rc = BV_OK;
+    void *bufs[] = {dx, dy, dz, dcx, dcy, dcz, dacc, dnbr, dredo};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    return rc;
+}
+
 extern "C" int bv_partition_gpu(int32_t device, int64_t n, const double *x, const double *y,
                                 const double *z, const double *obs, int32_t bc, int32_t cs,
                                 int32_t seed, int32_t epochs, int32_t time_flag, int32_t metric,
@@ -1890,9 +1937,16 @@ extern "C" int bv_partition_gpu(int32_t device, int64_t n, const double *x, cons
     const bool km = n >= 2 * (int64_t)bc; /* else classic Vecchia: no k-means */
     int rc = BV_OK;
     if (km) rc = bv_km_setup(D, n, x, y, z, time_flag, bc);
+    else if (hipStreamCreateWithFlags(&D.st, hipStreamNonBlocking) != hipSuccess)
+        rc = BV_ERR_HIP;
+    /* the kNN on the device unless BV_KNN_HOST=1 (great-circle distances stay
+       on the host: their library sin/cos/asin round differently) */
+    const char *kh = getenv("BV_KNN_HOST");
+    const bool dev_knn = !(kh && atoi(kh) != 0);
     if (rc == BV_OK)
         rc = bv_partition_run(n, x, y, z, obs, bc, cs, seed, epochs, time_flag, metric, threads,
-                              km ? bv_km_assign : nullptr, &D, out);
+                              km ? bv_km_assign : nullptr, &D, out,
+                              dev_knn ? bv_knn_dev : nullptr);
     bv_km_free(D);
     return rc;
 }

@@ -1815,6 +1815,167 @@ hipError_t bv_launch_sim_finish(unsigned nblk, hipStream_t st, const bv_block_de
     return hipGetLastError();
 }
 
+/* findNearestPoints (include/nearest_neighbor.h:76-148) on the device: for
+   block b >= 1 (one workgroup each), the cs points among positions
+   0..accum[b]-1 of the block-ordered points nearest to the block's centroid,
+   in the order the reference's selection sort leaves them.  Its step s takes
+   the first minimum (strict <) of the distances at positions s..N-1 and
+   swaps it to position s, so the result is: repeatedly the (distance,
+   current position)-smallest remaining point, where a swap moves the point
+   that sat at position s to the taken point's old position.  Only points
+   with distance <= the cs-th smallest can be taken; the kernel finds that
+   distance by a radix select over the distances' IEEE bits (non-negative, so
+   ordered as integers), gathers those candidates in position order, and
+   replays the sort's swaps on them.  Distances are the host's expression,
+   products and sums rounded separately, with the correctly rounded square
+   root: sqrt(dx dx + dy dy), and for 3-D sqrt(d d + dz dz).  A block with
+   more than BV_KNN_CAND tied candidates sets its flag; the host redoes it. */
+#define BV_KNN_CAND 2048
+#define BV_KNN_BITS 11
+
+__device__ __forceinline__ double bv_knn_dist(const double *__restrict__ x,
+                                              const double *__restrict__ y,
+                                              const double *__restrict__ z, int64_t i, double cx,
+                                              double cy, double cz, int time_flag) {
+#pragma clang fp contract(off)
+    const double dx = x[i] - cx, dy = y[i] - cy;
+    double d = sqrt(dx * dx + dy * dy);
+    if (time_flag) {
+        const double dz = cz - z[i];
+        d = sqrt(d * d + dz * dz);
+    }
+    return d;
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+bv_knn_kernel(const double *__restrict__ x, const double *__restrict__ y,
+              const double *__restrict__ z, const int64_t *__restrict__ accum,
+              const double *__restrict__ bcx, const double *__restrict__ bcy,
+              const double *__restrict__ bcz, int k, int time_flag, int32_t *__restrict__ nbr,
+              int32_t *__restrict__ overflow) {
+    __shared__ uint32_t hist[4][1 << BV_KNN_BITS];
+    __shared__ double cdist[BV_KNN_CAND];
+    __shared__ int32_t cpos[BV_KNN_CAND];
+    __shared__ int32_t corig[BV_KNN_CAND];
+    __shared__ uint64_t s_prefix, s_mask;
+    __shared__ int32_t s_rem, s_ncand, s_wcount[4];
+    const int b = blockIdx.x + 1;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t N = accum[b];
+    const double cx = bcx[b], cy = bcy[b], cz = time_flag ? bcz[b] : 0.0;
+    if (N < k) { /* the reference's "not enough points": never after firstbatch */
+        if (tid == 0) overflow[b] = 1;
+        return;
+    }
+    if (tid == 0) { s_prefix = 0; s_mask = 0; s_rem = k; }
+    __syncthreads();
+    /* radix select of the k-th smallest key, BV_KNN_BITS bits per pass from
+       the top (the last pass takes the remaining low bits) */
+    for (int hi = 64; hi > 0; hi -= BV_KNN_BITS) {
+        const int width = hi >= BV_KNN_BITS ? BV_KNN_BITS : hi;
+        const int shift = hi - width;
+        const uint64_t dmask = ((uint64_t)1 << width) - 1;
+        for (int t = tid; t < 4 * (1 << BV_KNN_BITS); t += 256) (&hist[0][0])[t] = 0;
+        __syncthreads();
+        const uint64_t prefix = s_prefix, mask = s_mask;
+        for (int64_t i = tid; i < N; i += 256) {
+            const uint64_t key =
+                __builtin_bit_cast(uint64_t, bv_knn_dist(x, y, z, i, cx, cy, cz, time_flag));
+            if ((key & mask) == prefix) atomicAdd(&hist[wave][(key >> shift) & dmask], 1u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int rem = s_rem;
+            int digit = 0;
+            for (int dgt = 0; dgt <= (int)dmask; ++dgt) {
+                const int cnt =
+                    (int)(hist[0][dgt] + hist[1][dgt] + hist[2][dgt] + hist[3][dgt]);
+                if (rem <= cnt) { digit = dgt; break; }
+                rem -= cnt;
+            }
+            s_rem = rem;
+            s_prefix = prefix | ((uint64_t)digit << shift);
+            s_mask = mask | (dmask << shift);
+        }
+        __syncthreads();
+    }
+    const uint64_t kth = s_prefix; /* bits of the k-th smallest distance */
+    /* candidates (key <= kth) in position order */
+    if (tid == 0) s_ncand = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < N; base += 256) {
+        const int64_t i = base + tid;
+        double d = 0.0;
+        bool c = false;
+        if (i < N) {
+            d = bv_knn_dist(x, y, z, i, cx, cy, cz, time_flag);
+            c = __builtin_bit_cast(uint64_t, d) <= kth;
+        }
+        const uint64_t bal = __ballot(c);
+        if (lane == 0) s_wcount[wave] = __popcll(bal);
+        __syncthreads();
+        int off = s_ncand;
+        for (int w = 0; w < wave; ++w) off += s_wcount[w];
+        off += __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+        if (c && off < BV_KNN_CAND) {
+            cdist[off] = d;
+            cpos[off] = (int32_t)i;
+            corig[off] = (int32_t)i;
+        }
+        __syncthreads();
+        if (tid == 0) s_ncand += s_wcount[0] + s_wcount[1] + s_wcount[2] + s_wcount[3];
+        __syncthreads();
+    }
+    const int nc = s_ncand;
+    if (nc > BV_KNN_CAND) {
+        if (tid == 0) overflow[b] = 1;
+        return;
+    }
+    if (wave != 0) return;
+    /* the selection sort's steps, replayed on the candidates by one wavefront;
+       a taken candidate's position becomes < every later step, so it is
+       marked by cpos = -1 - s instead */
+    int32_t *out = nbr + (int64_t)b * k;
+    for (int st = 0; st < k; ++st) {
+        double bd = 0.0;
+        int bp = INT32_MAX, bc = -1;
+        for (int c = lane; c < nc; c += 64) {
+            const int p = cpos[c];
+            if (p < 0) continue;
+            const double d = cdist[c];
+            if (bc < 0 || d < bd || (d == bd && p < bp)) { bd = d; bp = p; bc = c; }
+        }
+        /* wavefront argmin of (distance, position) */
+        for (int o = 32; o >= 1; o >>= 1) {
+            const double od = __shfl_xor(bd, o, 64);
+            const int op = __shfl_xor(bp, o, 64), oc = __shfl_xor(bc, o, 64);
+            if (oc >= 0 && (bc < 0 || od < bd || (od == bd && op < bp))) {
+                bd = od; bp = op; bc = oc;
+            }
+        }
+        const int from = bp;
+        /* the candidate sitting at position st moves to `from` */
+        for (int c = lane; c < nc; c += 64)
+            if (c != bc && cpos[c] == st) cpos[c] = from;
+        bv_wave_sync();
+        if (lane == 0) {
+            cpos[bc] = -1 - st;
+            out[st] = corig[bc];
+        }
+        bv_wave_sync();
+    }
+}
+
+hipError_t bv_launch_knn(const double *x, const double *y, const double *z, const int64_t *accum,
+                         const double *bcx, const double *bcy, const double *bcz, int64_t B,
+                         int k, int time_flag, int32_t *nbr, int32_t *overflow,
+                         hipStream_t st) {
+    if (B <= 1) return hipSuccess;
+    hipLaunchKernelGGL(bv_knn_kernel, dim3((unsigned)(B - 1)), dim3(256), 0, st, x, y, z, accum,
+                       bcx, bcy, bcz, k, time_flag, nbr, overflow);
+    return hipGetLastError();
+}
+
 /* The k-means assignment step of the host pre-processing (bv_prep.cpp) on
    the device: one thread per point, the centroids staged through LDS in
    tiles, the host's rounding (d = ((0 + dx*dx) + dy*dy) + dz*dz with

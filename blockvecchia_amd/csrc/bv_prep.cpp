@@ -241,7 +241,7 @@ extern "C" int bv_partition(int64_t n, const double *x, const double *y, const d
 int bv_partition_run(int64_t n, const double *x, const double *y, const double *z,
                      const double *obs, int32_t bc, int32_t cs, int32_t seed, int32_t epochs,
                      int32_t time_flag, int32_t metric, int32_t threads, bv_assign_fn assign,
-                     void *user, bv_blocks *out) {
+                     void *user, bv_blocks *out, bv_knn_fn knn) {
     if (!out || n <= 0 || !x || !y || !obs || bc <= 0 || cs < 0 || epochs < 0) return BV_ERR_ARG;
     if (time_flag && !z) return BV_ERR_ARG;
     if (n > 2147483647LL) return BV_ERR_ARG;
@@ -384,10 +384,25 @@ int bv_partition_run(int64_t n, const double *x, const double *y, const double *
         const double *bx = out->x, *by = out->y, *bz = out->z;
         int32_t *nbr = out->nbr;
         const int fc = firstCount;
+        std::vector<int32_t> redo((size_t)B, 1);
+        if (knn && metric == 0 && B > 1) {
+            /* block b's centroid is the permuted centroid b + firstCount - 1 */
+            std::vector<double> bcx((size_t)B, 0.0), bcy((size_t)B, 0.0), bcz((size_t)B, 0.0);
+            for (int64_t i = 1; i < B; ++i) {
+                bcx[i] = lx[i + fc - 1];
+                bcy[i] = ly[i + fc - 1];
+                bcz[i] = time_flag ? lz[i + fc - 1] : 0.0;
+            }
+            std::fill(redo.begin(), redo.end(), 0);
+            const int rc = knn(user, n, bx, by, bz, B, accum.data(), bcx.data(), bcy.data(),
+                               bcz.data(), cs, time_flag, nbr, redo.data());
+            if (rc != BV_OK) { bv_blocks_free(out); return rc; }
+        }
         parallel_for(B - 1, threads, [&](int64_t lo, int64_t hi) {
             std::vector<double> dist, tmp;
             for (int64_t ii = lo; ii < hi; ++ii) {
                 const int64_t i = ii + 1;
+                if (!redo[i]) continue;
                 const int64_t ci = i + fc - 1;
                 knn_block(bx, by, bz, accum[i], lx[ci], ly[ci], time_flag ? lz[ci] : 0.0, cs,
                           metric, time_flag, nbr + i * cs, dist, tmp);

@@ -16,11 +16,23 @@
 typedef int (*bv_assign_fn)(void *user, int k, const double *cx, const double *cy,
                             const double *cz, int32_t *cluster);
 
-/* bv_partition with the assignment step supplied by `assign` (nullptr: the
- * host loop).  The result does not depend on which one runs. */
+/* The conditioning sets of blocks 1..B-1 (findNearestPoints, Euclidean
+ * distance): for block b, nbr[b*cs .. b*cs+cs) = the positions among
+ * 0..accum[b]-1 of the block-ordered points (x, y, z) nearest to the block's
+ * centroid (cx[b], cy[b], cz[b]) in the reference's selection-sort order.
+ * Blocks it could not finish are flagged in redo[b] (1) and left to the host.
+ * Returns BV_OK or an error code. */
+typedef int (*bv_knn_fn)(void *user, int64_t n, const double *x, const double *y,
+                         const double *z, int64_t B, const int64_t *accum, const double *cx,
+                         const double *cy, const double *cz, int32_t cs, int32_t time_flag,
+                         int32_t *nbr, int32_t *redo);
+
+/* bv_partition with the assignment step supplied by `assign` and the
+ * Euclidean conditioning sets by `knn` (nullptr: the host loops).  The result
+ * does not depend on which one runs. */
 int bv_partition_run(int64_t n, const double *x, const double *y, const double *z,
                      const double *obs, int32_t bc, int32_t cs, int32_t seed, int32_t epochs,
                      int32_t time_flag, int32_t metric, int32_t threads, bv_assign_fn assign,
-                     void *user, bv_blocks *out);
+                     void *user, bv_blocks *out, bv_knn_fn knn = nullptr);
 
 #endif /* BV_PREP_H */
