@@ -35,6 +35,7 @@ hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t s
 hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,
                          const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
 hipError_t bv_launch_reduce(unsigned nevals, hipStream_t st, const bv_eval_slot *slots);
+hipError_t bv_launch_cheb_build(unsigned nslots, hipStream_t st, const bv_eval_slot *slots);
 hipError_t bv_launch_step(int kind, unsigned nitems, hipStream_t st,
                           const bv_step_item *items, const bv_block_desc *desc,
                           const bv_eval_slot *slots, double *ws, double *part_logdet,
@@ -96,6 +97,7 @@ struct bv_ctx {
     bv_eval_slot *d_slots = nullptr, *h_slots = nullptr;
     double *d_ws = nullptr;
     double *d_logdet = nullptr, *d_dot = nullptr, *d_result = nullptr, *h_result = nullptr;
+    double *d_cheb = nullptr;               /* general-nu tables, one per evaluation */
     int32_t *d_info = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -155,6 +157,8 @@ static void bv_free_eval_buffers(bv_ctx *c) {
     if (c->d_part_logdet) (void)hipFree(c->d_part_logdet);
     if (c->d_part_dot) (void)hipFree(c->d_part_dot);
     if (c->d_part_info) (void)hipFree(c->d_part_info);
+    if (c->d_cheb) (void)hipFree(c->d_cheb);
+    c->d_cheb = nullptr;
     c->d_items = nullptr; c->d_part_logdet = nullptr; c->d_part_dot = nullptr;
     c->d_part_info = nullptr;
     c->d_desc = nullptr; c->d_slots = nullptr; c->h_slots = nullptr; c->d_ws = nullptr;
@@ -181,6 +185,7 @@ static int bv_reserve(bv_ctx *c, int k) {
     BV_HIP(hipMalloc((void **)&c->d_part_logdet, sizeof(double) * B * k * c->pmax));
     BV_HIP(hipMalloc((void **)&c->d_part_dot, sizeof(double) * B * k * c->pmax));
     BV_HIP(hipMalloc((void **)&c->d_part_info, sizeof(int32_t) * B * k * c->pmax));
+    BV_HIP(hipMalloc((void **)&c->d_cheb, sizeof(double) * BV_CHEB_SIZE * k));
     c->cap_evals = k;
     return BV_OK;
 }
@@ -500,7 +505,9 @@ static int bv_enqueue(bv_ctx *c, unsigned nblk, int kinds_used) {
     const size_t lds = sizeof(double) * 4 * (size_t)c->max_mp;
     for (int kind = 0; kind < 8; ++kind) {
         if (!(kinds_used & (1 << kind))) continue;
-        if (kind == BV_KIND_GENERAL || kind == BV_KIND_SPACETIME) {
+        if (kind == BV_KIND_GENERAL)
+            BV_HIP(bv_launch_cheb_build((unsigned)c->desc_evals, c->stream, c->d_slots));
+        if (kind == BV_KIND_SPACETIME) {
             const unsigned T = (unsigned)(c->max_mp / BV_TILE);
             BV_HIP(bv_launch_gen(kind, T * (T + 1) / 2, nblk, c->stream, c->d_desc, c->d_slots,
                                  c->d_ws));
@@ -633,6 +640,7 @@ static int bv_run_once(bv_ctx *c, int k, int nparam, const double *thetas, doubl
         s.info = c->d_info + e * B;
         s.result = c->d_result + 4 * e;
         s.log2pi = log(2 * 3.141592653589793);
+        s.kp.cheb = c->d_cheb + (size_t)e * BV_CHEB_SIZE;
     }
     BV_HIP(hipEventRecord(c->ev0, c->stream));
     /* one kind per launch: evaluations of a call share the kernel when their
@@ -998,10 +1006,26 @@ extern "C" int bv_matern(int m, int nn, const double *x1, const double *y1, cons
         BV_HIP(hipMemcpy(d[2], z1, sm, hipMemcpyHostToDevice));
         BV_HIP(hipMemcpy(d[5], z2, sn, hipMemcpyHostToDevice));
     }
+    double *d_cheb = nullptr;
+    bv_eval_slot *d_sl = nullptr;
+    if (kind == BV_KIND_GENERAL) {
+        /* the evaluation's x^nu K_nu table, as the likelihood builds it */
+        BV_HIP(hipMalloc(&d_cheb, sizeof(double) * BV_CHEB_SIZE));
+        BV_HIP(hipMalloc(&d_sl, sizeof(bv_eval_slot)));
+        kp.cheb = d_cheb;
+        bv_eval_slot sl;
+        memset(&sl, 0, sizeof(sl));
+        sl.kp = kp;
+        sl.kind = kind;
+        BV_HIP(hipMemcpy(d_sl, &sl, sizeof(sl), hipMemcpyHostToDevice));
+        BV_HIP(bv_launch_cheb_build(1u, 0, d_sl));
+    }
     BV_HIP(bv_launch_matern(m, nn, d[0], d[1], d[2], d[3], d[4], d[5], kp, kind, d[6]));
     BV_HIP(hipMemcpy(A, d[6], sm * nn, hipMemcpyDeviceToHost));
     for (double *p : d)
         if (p) (void)hipFree(p);
+    if (d_cheb) (void)hipFree(d_cheb);
+    if (d_sl) (void)hipFree(d_sl);
     return BV_OK;
 }
 
@@ -1079,6 +1103,7 @@ struct bv_lane {
     int32_t *d_part_info = nullptr;
     bv_eval_slot *d_slots = nullptr, *h_slots = nullptr;
     double *d_logdet = nullptr, *d_dot = nullptr, *d_result = nullptr, *h_result = nullptr;
+    double *d_cheb = nullptr;                /* general-nu tables, one per evaluation */
     int32_t *d_info = nullptr;
 };
 
@@ -1117,6 +1142,8 @@ static void bv_lane_free(bv_lane &L) {
     if (L.d_info) (void)hipFree(L.d_info);
     if (L.d_result) (void)hipFree(L.d_result);
     if (L.h_result) (void)hipHostFree(L.h_result);
+    if (L.d_cheb) (void)hipFree(L.d_cheb);
+    L.d_cheb = nullptr;
     L.d_ws = nullptr; L.d_desc = nullptr; L.h_desc = nullptr; L.d_items = nullptr;
     L.h_items = nullptr; L.d_part_logdet = nullptr; L.d_part_dot = nullptr;
     L.d_part_info = nullptr; L.d_slots = nullptr;
@@ -1155,6 +1182,7 @@ static int bv_lane_reserve(bv_lane &L, int64_t ws, int64_t blocks, int evals, in
     BV_HIP(hipMalloc((void **)&L.d_info, sizeof(int32_t) * (size_t)blocks));
     BV_HIP(hipMalloc((void **)&L.d_result, sizeof(double) * 4 * evals));
     BV_HIP(hipHostMalloc((void **)&L.h_result, sizeof(double) * 4 * evals, hipHostMallocDefault));
+    BV_HIP(hipMalloc((void **)&L.d_cheb, sizeof(double) * BV_CHEB_SIZE * evals));
     L.cap_ws = ws; L.cap_blocks = blocks; L.cap_evals = evals;
     L.cap_items = nitems; L.cap_pmax = pmax;
     return BV_OK;
@@ -1287,6 +1315,7 @@ static int bv_server_launch_own(bv_server *s, bv_lane &L, const bv_request &r) {
     sl.info = c->d_info;
     sl.result = c->d_result;
     sl.log2pi = log(2 * 3.141592653589793);
+    sl.kp.cheb = c->d_cheb;
     BV_HIP(hipSetDevice(s->device));
     BV_HIP(hipMemcpyAsync(c->d_slots, L.h_slots, sizeof(bv_eval_slot), hipMemcpyHostToDevice,
                           c->stream));
@@ -1370,6 +1399,7 @@ static int bv_server_launch(bv_server *s, bv_lane &L) {
         sl.info = L.d_info + blk_off;
         sl.result = L.d_result + 4 * e;
         sl.log2pi = log(2 * 3.141592653589793);
+        sl.kp.cheb = L.d_cheb + (size_t)e * BV_CHEB_SIZE;
         form_of[e] = c->stepped;
         const int g = sl.kind * 4 + c->stepped;
         gcount[g] += c->B;
@@ -1419,12 +1449,15 @@ static int bv_server_launch(bv_server *s, bv_lane &L) {
         BV_HIP(hipMemcpyAsync(L.d_items, L.h_items, sizeof(bv_step_item) * (size_t)nitems,
                               hipMemcpyHostToDevice, L.stream));
     const int pst = L.cap_pmax; /* partial-sum stride */
+    bool general = false;
+    for (int g = 0; g < NG; ++g) general = general || (gcount[g] && g / 4 == BV_KIND_GENERAL);
+    if (general) BV_HIP(bv_launch_cheb_build((unsigned)k, L.stream, L.d_slots));
     for (int g = 0; g < NG; ++g) {
         const unsigned nblk = (unsigned)gcount[g];
         if (!nblk) continue;
         const int kd = g / 4, form = g % 4;
         const bv_block_desc *dd = L.d_desc + start[g];
-        if (kd == BV_KIND_GENERAL || kd == BV_KIND_SPACETIME) {
+        if (kd == BV_KIND_SPACETIME) {
             const unsigned T = (unsigned)(gmax_mp[g] / BV_TILE);
             BV_HIP(bv_launch_gen(kd, T * (T + 1) / 2, nblk, L.stream, dd, L.d_slots, L.d_ws));
         }
@@ -1621,9 +1654,13 @@ extern "C" int bv_simulate(bv_ctx *c, unsigned n, const double *theta, const dou
     sl.result = c->d_result;
     sl.log2pi = log(2 * 3.141592653589793);
     sl.flags = BV_SLOT_SAMPLE;
+    sl.kp.cheb = c->d_cheb;
     c->h_slots[0] = sl;
     if (hipMemcpyAsync(c->d_slots, c->h_slots, sizeof(bv_eval_slot), hipMemcpyHostToDevice,
                        c->stream) != hipSuccess)
+        return done(BV_ERR_HIP);
+    if (sl.kind == BV_KIND_GENERAL &&
+        bv_launch_cheb_build(1u, c->stream, c->d_slots) != hipSuccess)
         return done(BV_ERR_HIP);
     for (int l = 0; l < nlev; ++l) {
         const bv_block_desc *dd = d_desc + lstart[(size_t)l];
@@ -1631,7 +1668,7 @@ extern "C" int bv_simulate(bv_ctx *c, unsigned n, const double *theta, const dou
         if (!cnt) continue;
         const int mp = lmaxmp[(size_t)l];
         hipError_t e = hipSuccess;
-        if (sl.kind == BV_KIND_GENERAL || sl.kind == BV_KIND_SPACETIME) {
+        if (sl.kind == BV_KIND_SPACETIME) {
             const unsigned T = (unsigned)(mp / BV_TILE);
             e = bv_launch_gen(sl.kind, T * (T + 1) / 2, cnt, c->stream, dd, c->d_slots, c->d_ws);
         }
@@ -1768,7 +1805,8 @@ extern "C" int bv_llh_noalloc(const bv_problem *p, unsigned n, const double *the
     const size_t o_ld = o_desc + bv_align256(sizeof(bv_block_desc) * (size_t)B);
     const size_t o_dot = o_ld + bv_align256(sizeof(double) * (size_t)B);
     const size_t o_info = o_dot + bv_align256(sizeof(double) * (size_t)B);
-    const size_t o_ws = o_info + bv_align256(sizeof(int32_t) * (size_t)B);
+    const size_t o_cheb = o_info + bv_align256(sizeof(int32_t) * (size_t)B);
+    const size_t o_ws = o_cheb + bv_align256(sizeof(double) * BV_CHEB_SIZE);
     const size_t need = o_ws + sizeof(double) * (size_t)off;
     if (need > (size_t)BV_NOALLOC_SCRATCH_BYTES) {
         fprintf(stderr, "libbv: bv_llh_noalloc needs %zu bytes of scratch for this dataset, more "
@@ -1796,13 +1834,15 @@ extern "C" int bv_llh_noalloc(const bv_problem *p, unsigned n, const double *the
     slot.info = (int32_t *)(base + o_info);
     slot.result = d_res;
     slot.log2pi = log(2 * 3.141592653589793);
+    slot.kp.cheb = (const double *)(base + o_cheb);
     const hipStream_t st = 0;
     BV_HIP(hipMemcpyAsync(d_slot, &slot, sizeof(slot), hipMemcpyHostToDevice, st));
     BV_HIP(hipMemcpyAsync(d_bn, p->batch_num, sizeof(int32_t) * (size_t)B, hipMemcpyHostToDevice,
                           st));
     BV_HIP(hipMemcpyAsync(d_desc, desc.data(), sizeof(bv_block_desc) * (size_t)B,
                           hipMemcpyHostToDevice, st));
-    if (slot.kind == BV_KIND_GENERAL || slot.kind == BV_KIND_SPACETIME) {
+    if (slot.kind == BV_KIND_GENERAL) BV_HIP(bv_launch_cheb_build(1u, st, d_slot));
+    if (slot.kind == BV_KIND_SPACETIME) {
         const unsigned T = (unsigned)(max_mp / BV_TILE);
         BV_HIP(bv_launch_gen(slot.kind, T * (T + 1) / 2, (unsigned)B, st, d_desc, d_slot, d_ws));
     }

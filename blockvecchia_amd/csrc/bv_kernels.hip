@@ -1815,6 +1815,37 @@ hipError_t bv_launch_sim_finish(unsigned nblk, hipStream_t st, const bv_block_de
     return hipGetLastError();
 }
 
+/* The general-nu table of every evaluation slot of a call (bv_xnu_knu_cheb):
+   one workgroup per (half-octave, slot); slots of other kinds are left
+   alone.  The node values g = x^nu e^x K_nu(x) come from the accurate
+   routine, the coefficients from their cosine sums.  Written through
+   kp.cheb, the slot's own table. */
+extern "C" __global__ void __launch_bounds__(64)
+bv_cheb_build_kernel(const bv_eval_slot *__restrict__ slots) {
+    const bv_eval_slot &E = slots[blockIdx.y];
+    if (E.kind != BV_KIND_GENERAL) return;
+    __shared__ double gv[BV_CHEB_N];
+    const int j = blockIdx.x, t = threadIdx.x;
+    const double nu = E.kp.nu;
+    if (t < BV_CHEB_N) {
+        const double x = bv_cheb_node_x(j, t);
+        gv[t] = exp(nu * log(x)) * bv_besselk_scaled(nu, x);
+    }
+    __syncthreads();
+    if (t < BV_CHEB_N) {
+        double sum = 0.0;
+        for (int k = 0; k < BV_CHEB_N; ++k) sum += gv[k] * cos(BV_PI * t * (k + 0.5) / BV_CHEB_N);
+        sum *= 2.0 / BV_CHEB_N;
+        ((double *)E.kp.cheb)[j * BV_CHEB_N + t] = (t == 0) ? 0.5 * sum : sum;
+    }
+}
+
+hipError_t bv_launch_cheb_build(unsigned nslots, hipStream_t st, const bv_eval_slot *slots) {
+    if (nslots == 0) return hipSuccess;
+    hipLaunchKernelGGL(bv_cheb_build_kernel, dim3(BV_CHEB_NJ, nslots), dim3(64), 0, st, slots);
+    return hipGetLastError();
+}
+
 /* findNearestPoints (include/nearest_neighbor.h:76-148) on the device: for
    block b >= 1 (one workgroup each), the cs points among positions
    0..accum[b]-1 of the block-ordered points nearest to the block's centroid,
@@ -2068,7 +2099,7 @@ bv_matern_kernel(int m, int n, const double *x1, const double *y1, const double 
                  const double *x2, const double *y2, const double *z2, bv_kparams kp, int kind,
                  double *A) {
     __shared__ bv_knu_tab s_tab;
-    if (kind == BV_KIND_GENERAL || kind == BV_KIND_SPACETIME) {
+    if (kind == BV_KIND_SPACETIME) {
         bv_knu_tab_fill(kp.nu, &s_tab, (int)threadIdx.x, (int)blockDim.x);
         __syncthreads();
     }
@@ -2085,7 +2116,7 @@ bv_matern_kernel(int m, int n, const double *x1, const double *y1, const double 
         v = bv_cov_tab<BV_KIND_SPACETIME>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp, s_tab);
         break;
     default:
-        v = bv_cov_tab<BV_KIND_GENERAL>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp, s_tab);
+        v = bv_cov<BV_KIND_GENERAL>(x1[i], y1[i], za, x2[jj], y2[jj], zb, kp);
         break;
     }
     A[e] = v;
@@ -2117,6 +2148,10 @@ hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t s
         hipLaunchKernelGGL(bv_block_kernel_1w<BV_KIND_NU25>, dim3(nblk), dim3(64), lds, st, desc,
                            slots, ws);
         break;
+    case BV_KIND_GENERAL:
+        hipLaunchKernelGGL(bv_block_kernel_1w<BV_KIND_GENERAL>, dim3(nblk), dim3(64), lds, st,
+                           desc, slots, ws);
+        break;
     default:
         hipLaunchKernelGGL(bv_block_kernel_1w<BV_KIND_LOADA>, dim3(nblk), dim3(64), 0, st, desc,
                            slots, ws);
@@ -2141,6 +2176,10 @@ static hipError_t bv_launch_block_nw(int kind, unsigned nblk, size_t lds, hipStr
         break;
     case BV_KIND_NU25:
         hipLaunchKernelGGL((bv_block_kernel<BV_KIND_NU25, NW>), dim3(nblk), blk, lds, st, desc,
+                           slots, ws);
+        break;
+    case BV_KIND_GENERAL:
+        hipLaunchKernelGGL((bv_block_kernel<BV_KIND_GENERAL, NW>), dim3(nblk), blk, lds, st, desc,
                            slots, ws);
         break;
     default:
@@ -2196,6 +2235,10 @@ hipError_t bv_launch_step(int kind, unsigned nitems, hipStream_t st,
         hipLaunchKernelGGL(bv_step_kernel<BV_KIND_NU25>, dim3(nitems), dim3(64), 0, st, items,
                            (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
+    case BV_KIND_GENERAL:
+        hipLaunchKernelGGL(bv_step_kernel<BV_KIND_GENERAL>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
     default:
         hipLaunchKernelGGL(bv_step_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(64), 0, st, items,
                            (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
@@ -2222,6 +2265,10 @@ hipError_t bv_launch_step2(int kind, unsigned nitems, hipStream_t st,
         hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_NU25>, dim3(nitems), dim3(64), 0, st, items,
                            (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
+    case BV_KIND_GENERAL:
+        hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_GENERAL>, dim3(nitems), dim3(64), 0, st, items,
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
     default:
         hipLaunchKernelGGL(bv_step2_kernel<BV_KIND_LOADA>, dim3(nitems), dim3(64), 0, st, items,
                            (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
@@ -2246,6 +2293,10 @@ hipError_t bv_launch_pair(int kind, unsigned nitems, hipStream_t st,
         break;
     case BV_KIND_NU25:
         hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_NU25>, dim3(nitems), dim3(128), 0, st, items,
+                           (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
+        break;
+    case BV_KIND_GENERAL:
+        hipLaunchKernelGGL(bv_pair_kernel<BV_KIND_GENERAL>, dim3(nitems), dim3(128), 0, st, items,
                            (int)nitems, desc, slots, ws, part_logdet, part_dot, part_info, pmax);
         break;
     default:

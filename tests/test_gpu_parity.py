@@ -59,6 +59,28 @@ def test_matern_entries(theta, kernel, metric, ds, use_z):
     np.testing.assert_allclose(A, R, rtol=1e-13, atol=1e-15 * theta[0])
 
 
+@pytest.mark.parametrize("nu", [0.01, 0.3, 0.8, 1.0, 1.2, 2.0, 2.7, 3.0])
+def test_matern_general_nu_wide_range(nu):
+    """General nu through the per-evaluation Chebyshev table of x^nu K_nu(x)
+    e^x (half-octaves from 2^-24 to 2^10, the accurate routine below, 0
+    above) against the oracle's quadrature: scaled distances from 1e-9 to
+    900 (every half-octave, both interval halves, the boundaries, below and
+    above the table), 1e-13 relative."""
+    _say("case: general nu=%g over x in [1e-9, 900]" % nu)
+    beta = 0.5
+    xs = np.concatenate([np.geomspace(1e-9, 900.0, 700),
+                         2.0 ** np.arange(-26, 11, 0.5), [2.0 ** -24 * (1 - 1e-12), 1023.9]])
+    d = xs * beta
+    x1, y1 = np.zeros(1), np.zeros(1)
+    x2, y2 = d, np.zeros(len(d))
+    th = [1.3, beta, nu]
+    A = bv.matern(x1, y1, x2, y2, th)[0]
+    R = oracle.matern(x1, y1, x2, y2, th)[0]
+    tiny = R < 1e-290
+    np.testing.assert_allclose(A[~tiny], R[~tiny], rtol=1e-13, atol=0)
+    assert np.all(np.abs(A[tiny]) < 1e-280)
+
+
 def test_spacetime_entries():
     _say("case: space-time entries")
     rng = np.random.default_rng(6)
