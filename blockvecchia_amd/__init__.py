@@ -80,7 +80,7 @@ class BvBlocks(ctypes.Structure):
 EXPORTED = [
     "bv_create", "bv_llh", "llh_Xvecchia_batch", "bv_llh_many", "bv_block_terms",
     "bv_last_status", "bv_last_kernel_ms", "bv_iterations", "bv_destroy", "bv_matern",
-    "bv_partition", "bv_blocks_free", "bv_device_selftest",
+    "bv_partition", "bv_blocks_free", "bv_device_selftest", "bv_trace_reset", "bv_trace_read",
     "bv_server_create", "bv_server_attach", "bv_server_submit", "bv_server_poll",
     "bv_server_pending", "bv_server_destroy", "bv_factor_config", "bv_set_graph",
     "bv_partition_gpu", "bv_factor_candidates", "bv_llh_noalloc", "bv_simulate",

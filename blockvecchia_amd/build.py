@@ -1,6 +1,7 @@
 """Build recipe for libbv.so (gfx950 kernels + host engine + host
 pre-processing) and the test_dvecchia_batch driver, with hipcc and g++
-directly.  Usage: python -m blockvecchia_amd.build  (or build_all())."""
+directly.  Usage: python -m blockvecchia_amd.build  (or build_all());
+python -m blockvecchia_amd.build --trace builds lib/libbv_trace.so only."""
 import os
 import subprocess
 import sys
@@ -29,19 +30,24 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_lib():
+def build_lib(trace=False):
+    """libbv.so, or with trace=True libbv_trace.so: the same library whose
+    factorization kernels record every wavefront's start and end
+    (bv_trace_read in include/bv.h)."""
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
+    tag = "_trace" if trace else ""
     objs = []
     for src in ["bv_kernels.hip", "bv_api.hip"]:
-        obj = os.path.join(OBJDIR, src + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        obj = os.path.join(OBJDIR, src + tag + ".o")
+        extra = ["-DBV_TRACE"] if trace and src == "bv_kernels.hip" else []
+        _run([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in ["bv_prep.cpp"]:
         obj = os.path.join(OBJDIR, src + ".o")
         _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    out = os.path.join(LIBDIR, "libbv.so")
+    out = os.path.join(LIBDIR, "libbv%s.so" % tag)
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, out)
@@ -66,5 +72,8 @@ def build_all():
 
 
 if __name__ == "__main__":
-    build_all()
+    if "--trace" in sys.argv[1:]:
+        build_lib(trace=True)
+    else:
+        build_all()
     sys.exit(0)

@@ -38,6 +38,40 @@
 
 typedef double bv_d4 __attribute__((ext_vector_type(4)));
 
+/* Wavefront trace (builds with BV_TRACE only; see bv_trace_read in bv.h):
+   each wavefront of a factorization kernel appends {start, end, id} when it
+   leaves the kernel, start and end from the 100 MHz real-time counter, id =
+   (kernel << 56) | ((step + 1) << 48) | (workgroup << 8) | wavefront.  The
+   scope object's destructor runs on every return path. */
+#ifdef BV_TRACE
+#define BV_TRACE_CAP (1u << 22)
+__device__ unsigned long long bv_trace_rec[3ull * BV_TRACE_CAP];
+__device__ unsigned int bv_trace_n;
+struct bv_trace_wave {
+    unsigned long long t0, id;
+    __device__ __forceinline__ bv_trace_wave(int kernel, int step) {
+        t0 = __builtin_amdgcn_s_memrealtime();
+        id = ((unsigned long long)(unsigned)kernel << 56) |
+             ((unsigned long long)(unsigned)((step + 1) & 0xff) << 48) |
+             ((unsigned long long)blockIdx.x << 8) | (threadIdx.x >> 6);
+    }
+    __device__ __forceinline__ ~bv_trace_wave() {
+        if ((threadIdx.x & 63) == 0) {
+            const unsigned long long t1 = __builtin_amdgcn_s_memrealtime();
+            const unsigned int i = atomicAdd(&bv_trace_n, 1u);
+            if (i < BV_TRACE_CAP) {
+                bv_trace_rec[3ull * i] = t0;
+                bv_trace_rec[3ull * i + 1] = t1;
+                bv_trace_rec[3ull * i + 2] = id;
+            }
+        }
+    }
+};
+#define BV_TRACE_WAVE(kernel, step) bv_trace_wave bv_trace_scope_((kernel), (step))
+#else
+#define BV_TRACE_WAVE(kernel, step) ((void)0)
+#endif
+
 #define BV_KIND_LOADA 7 /* factor kernel reads A from the workspace tiles */
 
 __device__ __forceinline__ bv_d4 bv_mfma(double a, double b, bv_d4 c) {
@@ -527,6 +561,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
     __shared__ bv_kparams s_kp; /* read where used: not held in registers */
 
     const bv_block_desc d = desc[blockIdx.x];
+    BV_TRACE_WAVE(0, -1);
     const bv_eval_slot &E = slots[d.eval];
     const int tid0 = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid0 >> 6); /* wave-uniform */
@@ -653,6 +688,7 @@ bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *_
     __shared__ bv_kparams s_kp;
 
     const bv_block_desc d = desc[blockIdx.x];
+    BV_TRACE_WAVE(0, -1);
     const bv_eval_slot &E = slots[d.eval];
     const int lane0 = threadIdx.x;
     const int m = d.m, cs0 = d.cs0, Mp = d.mp;
@@ -833,6 +869,7 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems,
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
     if (it.desc < 0) return;
+    BV_TRACE_WAVE(1, it.step);
     const int j = it.step;
     const bv_block_desc d = desc[it.desc];
     const bv_eval_slot &E = slots[d.eval];
@@ -1085,6 +1122,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
     if (it.desc < 0) return;
+    BV_TRACE_WAVE(2, it.step);
     const int j = it.step;
     const bv_block_desc d = desc[it.desc];
     const bv_eval_slot &E = slots[d.eval];
@@ -1481,6 +1519,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
     if (it.desc < 0) return;
+    BV_TRACE_WAVE(3, it.step);
     const int s = it.step;
     const bv_block_desc d = desc[it.desc];
     const bv_eval_slot &E = slots[d.eval];
@@ -2342,3 +2381,38 @@ hipError_t bv_launch_kmeans_assign(int64_t n, const double *px, const double *py
                        cx, cy, cz, cluster);
     return hipGetLastError();
 }
+
+/* Wavefront trace readers (bv.h). */
+#include "bv.h"
+#ifdef BV_TRACE
+extern "C" int bv_trace_reset(int device) {
+    if (hipSetDevice(device) != hipSuccess) return BV_ERR_HIP;
+    const unsigned int zero = 0;
+    return hipMemcpyToSymbol(HIP_SYMBOL(bv_trace_n), &zero, sizeof(zero)) == hipSuccess ? BV_OK
+                                                                                       : BV_ERR_HIP;
+}
+
+extern "C" int bv_trace_read(int device, uint64_t *records, int64_t capacity, int64_t *count) {
+    if (!records || !count || capacity < 0) return BV_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return BV_ERR_HIP;
+    unsigned int n = 0;
+    if (hipMemcpyFromSymbol(&n, HIP_SYMBOL(bv_trace_n), sizeof(n)) != hipSuccess) return BV_ERR_HIP;
+    int64_t k = n < BV_TRACE_CAP ? (int64_t)n : (int64_t)BV_TRACE_CAP;
+    if (k > capacity) k = capacity;
+    if (k > 0 && hipMemcpyFromSymbol(records, HIP_SYMBOL(bv_trace_rec),
+                                     (size_t)k * 3 * sizeof(uint64_t)) != hipSuccess)
+        return BV_ERR_HIP;
+    *count = n;
+    return BV_OK;
+}
+#else
+extern "C" int bv_trace_reset(int device) {
+    (void)device;
+    return BV_ERR_UNSUPPORTED;
+}
+
+extern "C" int bv_trace_read(int device, uint64_t *records, int64_t capacity, int64_t *count) {
+    (void)device; (void)records; (void)capacity; (void)count;
+    return BV_ERR_UNSUPPORTED;
+}
+#endif

@@ -299,6 +299,21 @@ void bv_blocks_free(bv_blocks *b);
  * current device (one 16x16x4 FP64 product against a host product). */
 int bv_device_selftest(int device);
 
+/* Wavefront trace of the factorization kernels, for launch-occupancy
+ * analysis.  In a library built with BV_TRACE (`python -m
+ * blockvecchia_amd.build --trace` writes lib/libbv_trace.so), every wavefront
+ * of the fused, stepped, lookahead and paired factorization kernels appends
+ * one record of three 64-bit words when it leaves the kernel: its start and
+ * end on the device's 100 MHz real-time counter, and (kernel << 56) |
+ * ((step + 1) << 48) | (workgroup << 8) | wavefront, kernel 0 fused, 1
+ * stepped, 2 lookahead, 3 paired; at most 4,194,304 records per device.
+ * bv_trace_reset empties the device's trace; bv_trace_read copies up to
+ * `capacity` records and sets *count to the number appended since the reset
+ * (which may exceed what was kept).  Other builds return BV_ERR_UNSUPPORTED.
+ * No counterpart in the reference. */
+int bv_trace_reset(int device);
+int bv_trace_read(int device, uint64_t *records, int64_t capacity, int64_t *count);
+
 #ifdef __cplusplus
 }
 #endif
