@@ -102,7 +102,7 @@ def calibration_copy(mib, repeats=3):
 FACTOR_PREFIXES = ("void bv_block_kernel<", "void bv_step_kernel<", "void bv_step2_kernel<",
                    "void bv_pair_kernel<",
                    "void bv_block_kernel_1w<", "bv_step_parts_kernel",
-                   "void bv_gen_kernel<")
+                   "void bv_gen_kernel<", "bv_cheb_build_kernel")
 
 
 def summarize_traffic(fetch_csv, write_csv, mib, config):
@@ -410,7 +410,8 @@ def main():
     theta = [1.5, 0.1, args.nu]
     closed = args.nu in (0.5, 1.5, 2.5)
     nu_desc = ("nu=%g closed form" % args.nu) if closed else \
-        ("nu=%g: bv_gen_kernel (device K_nu) + factorization" % args.nu)
+        ("nu=%g: x^nu K_nu(x) from a per-evaluation Chebyshev table (bv_cheb_build_kernel), "
+         "entries generated inside the factorization" % args.nu)
     F = flops.eval_flops(blocks.batch_num, cs)
     eng = bv.Engine(blocks)
     for _ in range(args.warmup):

@@ -1865,18 +1865,9 @@ bv_cheb_build_kernel(const bv_eval_slot *__restrict__ slots) {
     if (E.kind != BV_KIND_GENERAL) return;
     __shared__ double gv[BV_CHEB_N];
     const int j = blockIdx.x, t = threadIdx.x;
-    const double nu = E.kp.nu;
-    if (t < BV_CHEB_N) {
-        const double x = bv_cheb_node_x(j, t);
-        gv[t] = exp(nu * log(x)) * bv_besselk_scaled(nu, x);
-    }
+    if (t < BV_CHEB_N) gv[t] = bv_cheb_node_g(E.kp.nu, j, t);
     __syncthreads();
-    if (t < BV_CHEB_N) {
-        double sum = 0.0;
-        for (int k = 0; k < BV_CHEB_N; ++k) sum += gv[k] * cos(BV_PI * t * (k + 0.5) / BV_CHEB_N);
-        sum *= 2.0 / BV_CHEB_N;
-        ((double *)E.kp.cheb)[j * BV_CHEB_N + t] = (t == 0) ? 0.5 * sum : sum;
-    }
+    if (t < BV_CHEB_N) ((double *)E.kp.cheb)[j * BV_CHEB_N + t] = bv_cheb_coef(gv, t);
 }
 
 hipError_t bv_launch_cheb_build(unsigned nslots, hipStream_t st, const bv_eval_slot *slots) {

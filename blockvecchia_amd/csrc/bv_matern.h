@@ -337,6 +337,21 @@ BV_HD double bv_cheb_node_x(int j, int k) {
     return ldexp(u, e);
 }
 
+/* The table's input: g = x^nu e^x K_nu(x) at node k of half-octave j. */
+BV_HD double bv_cheb_node_g(double nu, int j, int k) {
+    const double x = bv_cheb_node_x(j, k);
+    return exp(nu * log(x)) * bv_besselk_scaled(nu, x);
+}
+
+/* Coefficient t of a half-octave's series from its BV_CHEB_N node values
+   (discrete cosine sum; the constant term halved). */
+BV_HD double bv_cheb_coef(const double *gv, int t) {
+    double sum = 0.0;
+    for (int k = 0; k < BV_CHEB_N; ++k) sum += gv[k] * cos(BV_PI * t * (k + 0.5) / BV_CHEB_N);
+    sum *= 2.0 / BV_CHEB_N;
+    return (t == 0) ? 0.5 * sum : sum;
+}
+
 /* x^nu K_nu(x), x > 0, from the table. */
 BV_HD double bv_xnu_knu_cheb(double x, const double *cheb, double nu) {
     if (!(x >= 0x1p-24)) return bv_xnu_knu_slow(x, nu);

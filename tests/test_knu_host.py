@@ -22,6 +22,17 @@ extern "C" double shim_xnu_knu_tab(double nu, double x) {
     bv_knu_tab_fill(nu, &t, 0, 1);
     return bv_xnu_knu_tab(x, t);
 }
+extern "C" int shim_cheb_size() { return BV_CHEB_SIZE; }
+extern "C" void shim_cheb_table(double nu, double *tab) {
+    double gv[BV_CHEB_N];
+    for (int j = 0; j < BV_CHEB_NJ; ++j) {
+        for (int k = 0; k < BV_CHEB_N; ++k) gv[k] = bv_cheb_node_g(nu, j, k);
+        for (int t = 0; t < BV_CHEB_N; ++t) tab[j * BV_CHEB_N + t] = bv_cheb_coef(gv, t);
+    }
+}
+extern "C" double shim_xnu_knu_cheb(double nu, double x, const double *tab) {
+    return bv_xnu_knu_cheb(x, tab, nu);
+}
 extern "C" double shim_cov(int kind, double x1, double y1, double x2, double y2, double s2,
                            double beta, double nu, double con) {
     bv_kparams kp = {};
@@ -31,7 +42,17 @@ extern "C" double shim_cov(int kind, double x1, double y1, double x2, double y2,
     case 0: return bv_cov<BV_KIND_NU05>(x1, y1, 0, x2, y2, 0, kp);
     case 1: return bv_cov<BV_KIND_NU15>(x1, y1, 0, x2, y2, 0, kp);
     case 2: return bv_cov<BV_KIND_NU25>(x1, y1, 0, x2, y2, 0, kp);
-    default: return bv_cov<BV_KIND_GENERAL>(x1, y1, 0, x2, y2, 0, kp);
+    default: {
+        /* the general kind reads the evaluation's x^nu K_nu table */
+        static double tab[BV_CHEB_SIZE];
+        static double tab_nu = -1.0;
+        if (tab_nu != nu) {
+            shim_cheb_table(nu, tab);
+            tab_nu = nu;
+        }
+        kp.cheb = tab;
+        return bv_cov<BV_KIND_GENERAL>(x1, y1, 0, x2, y2, 0, kp);
+    }
     }
 }
 '''
@@ -51,6 +72,11 @@ def shim(tmp_path_factory):
     lib.shim_besselk.restype = ctypes.c_double
     lib.shim_xnu_knu_tab.argtypes = [ctypes.c_double, ctypes.c_double]
     lib.shim_xnu_knu_tab.restype = ctypes.c_double
+    lib.shim_cheb_size.restype = ctypes.c_int
+    lib.shim_cheb_table.argtypes = [ctypes.c_double, ctypes.c_void_p]
+    lib.shim_cheb_table.restype = None
+    lib.shim_xnu_knu_cheb.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+    lib.shim_xnu_knu_cheb.restype = ctypes.c_double
     lib.shim_cov.argtypes = [ctypes.c_int] + [ctypes.c_double] * 8
     lib.shim_cov.restype = ctypes.c_double
     return lib
@@ -116,3 +142,62 @@ def test_tabulated_knu_golden_and_random(shim):
         a = shim.shim_xnu_knu_tab(nu, x)
         b = math.exp(nu * math.log(x)) * oracle.besselk(nu, x)
         assert abs(a - b) <= 2e-14 * b, (nu, x, a, b)
+
+
+def _cheb(shim, nu, cache={}):
+    if nu not in cache:
+        tab = np.zeros(shim.shim_cheb_size())
+        shim.shim_cheb_table(nu, tab.ctypes.data)
+        cache[nu] = tab
+    return cache[nu]
+
+
+def test_chebyshev_knu_golden_and_random(shim):
+    """x^nu K_nu(x) from the per-nu Chebyshev table (the path every general-nu
+    entry takes: 14 terms on each half-octave of [2^-24, 2^10), coefficients
+    from the accurate routine at the Chebyshev nodes) against the 40-digit
+    mpmath table and, on random (nu, x), the oracle's quadrature K_nu.  The
+    table's coefficients here are built on the host by the same functions the
+    device's bv_cheb_build_kernel calls."""
+    import math
+    with open(os.path.join(HERE, "golden", "besselk_golden.json")) as f:
+        rows = json.load(f)["rows"]
+    worst, n = 0.0, 0
+    for nu, x, ref in rows:
+        ref = float(ref)
+        if ref == 0.0 or nu == 0.0:
+            continue
+        r = math.exp(nu * math.log(x)) * ref
+        if r == 0.0 or not math.isfinite(r) or r < 1e-300:
+            continue
+        v = shim.shim_xnu_knu_cheb(nu, x, _cheb(shim, nu).ctypes.data)
+        worst = max(worst, abs(v - r) / r)
+        n += 1
+    assert n > 300
+    assert worst < 2e-14, worst
+    rng = np.random.default_rng(23)
+    for _ in range(300):
+        nu = float(rng.uniform(0.01, 3.0))
+        x = float(10 ** rng.uniform(-9, np.log10(650)))
+        tab = _cheb(shim, nu)
+        a = shim.shim_xnu_knu_cheb(nu, x, tab.ctypes.data)
+        b = math.exp(nu * math.log(x)) * oracle.besselk(nu, x)
+        assert abs(a - b) <= 2e-14 * b, (nu, x, a, b)
+
+
+def test_chebyshev_knu_half_integer_closed_forms(shim):
+    """For nu = 1/2, 3/2, 5/2 the table must reproduce the elementary forms
+    x^nu K_nu(x) = sqrt(pi/2) e^-x p(x), p = 1, 1 + x, 3 + 3x + x^2 (a
+    dropped or misplaced node or coefficient fails this on whole
+    half-octaves)."""
+    import math
+    forms = {0.5: lambda x: 1.0, 1.5: lambda x: 1.0 + x, 2.5: lambda x: 3.0 + 3.0 * x + x * x}
+    for nu, p in forms.items():
+        tab = _cheb(shim, nu)
+        for x in np.geomspace(2.0 ** -24, 700.0, 997):
+            x = float(x)
+            ref = math.sqrt(math.pi / 2) * math.exp(-x) * p(x)
+            if ref < 1e-300:
+                continue
+            v = shim.shim_xnu_knu_cheb(nu, x, tab.ctypes.data)
+            assert abs(v - ref) <= 1e-14 * ref, (nu, x, v, ref)
