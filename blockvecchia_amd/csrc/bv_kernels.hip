@@ -347,15 +347,12 @@ __device__ __forceinline__ bv_d4 bv_transpose16(const bv_d4 &v, int lane, double
 #define BV_SWEEP_DPP 1
 #endif
 
-/* The value of v in lane K of this lane's row of 16 lanes (DPP row_newbcast,
-   no LDS round trip). */
+/* The value of v in lane K of this lane's row of 16 lanes: one 64-bit DPP
+   move (row_newbcast writes every lane, so the destination needs no initial
+   value), no LDS round trip. */
 template <int K>
 __device__ __forceinline__ double bv_row_bcast(double v) {
-    const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x150 + K, 0xF, 0xF, false);
-    const int hi =
-        __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x150 + K, 0xF, 0xF, false);
-    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xF, 0xF, false);
 }
 
 /* Step K of the sweep below without LDS: the column-K factors of the other
@@ -387,17 +384,17 @@ __device__ __forceinline__ void bv_sweep_step(bv_d4 &a, bv_d4 &w, int lane, int 
     const double lrk = __shfl(t, lr + 16 * kq, 64);
     if (lr == kml && K < kml && q >= cs0 && lq == 0) dacc += lrk * lrk;
     if (lane == 0) sPiv[pivoff + K] = akk; /* l_kk^2: log-determinant term */
-    bv_d4 lk, wk;
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        lk[g] = bv_row_bcast<K>(a[g]) * pinv;
-        wk[g] = bv_row_bcast<K>(w[g]) * pinv;
-    }
+    /* W: row K scaled by 1/l_KK (for its own lanes the broadcast is the lane's
+       own value, so w * pinv there is exactly wk), the rows below it minus
+       l_rK times the scaled row */
     const double lre = (lr > K) ? lrk : 0.0;
+    const double ws = (lr == K) ? pinv : 1.0;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-        a[g] -= lrk * lk[g];
-        w[g] = (lr == K) ? wk[g] : w[g] - lre * wk[g];
+        const double lk = bv_row_bcast<K>(a[g]) * pinv;
+        const double wk = bv_row_bcast<K>(w[g]) * pinv;
+        a[g] -= lrk * lk;
+        w[g] = fma(-lre, wk, w[g] * ws);
     }
 }
 
