@@ -100,7 +100,7 @@ def calibration_copy(mib, repeats=3):
 
 
 FACTOR_PREFIXES = ("void bv_block_kernel<", "void bv_step_kernel<", "void bv_step2_kernel<",
-                   "void bv_pair_kernel<",
+                   "void bv_pair_kernel<", "void bv_res_kernel<",
                    "void bv_block_kernel_1w<", "bv_step_parts_kernel",
                    "void bv_gen_kernel<", "bv_cheb_build_kernel")
 
@@ -420,7 +420,10 @@ def main():
     if st != bv.BV_OK:
         raise SystemExit("evaluation failed: status %d block %d" % (st, bad))
     cfg = eng.factor_config()
-    if cfg is None or cfg[1] == 0:
+    if cfg is not None and cfg[1] == 0 and cfg[0] == 3:
+        kname = ("bv_res_kernel (register-resident factor, one workgroup of four wavefronts "
+                 "per block, %s%s)" % (nu_desc, ", HIP graph replay" if cfg[4] else ""))
+    elif cfg is None or cfg[1] == 0:
         nw = {0: "four wavefronts", 1: "one wavefront", 2: "two wavefronts"}[cfg[0] if cfg else 0]
         kname = "bv_block_kernel (%s, %s per block)" % (nu_desc, nw)
     elif cfg[1] == 3:

@@ -15,6 +15,9 @@ OBJDIR = os.path.join(PKG, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# covariance kinds of the register-resident kernels (bv_matern.h, bv_device.h):
+# nu = 1/2, 3/2, 5/2, general nu, A read from the workspace
+RES_KINDS = (0, 1, 2, 3, 7)
 ARCH = "gfx950"
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
@@ -30,6 +33,23 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+def _run_parallel(cmds, workers=None):
+    """Run independent compile commands at once (at most `workers`, default
+    the usable CPUs up to 8); raise if any fails."""
+    workers = workers or max(1, min(8, len(os.sched_getaffinity(0))))
+    pending, running, failed = list(cmds), [], []
+    while pending or running:
+        while pending and len(running) < workers:
+            cmd = pending.pop(0)
+            print("+ " + " ".join(cmd), flush=True)
+            running.append((cmd, subprocess.Popen(cmd)))
+        cmd, proc = running.pop(0)
+        if proc.wait() != 0:
+            failed.append(cmd)
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+
+
 def build_lib(trace=False):
     """libbv.so, or with trace=True libbv_trace.so: the same library whose
     factorization kernels record every wavefront's start and end
@@ -37,16 +57,21 @@ def build_lib(trace=False):
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
     tag = "_trace" if trace else ""
-    objs = []
+    jobs = []
     for src in ["bv_kernels.hip", "bv_api.hip"]:
         obj = os.path.join(OBJDIR, src + tag + ".o")
         extra = ["-DBV_TRACE"] if trace and src == "bv_kernels.hip" else []
-        _run([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append(([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
+    # the register-resident kernels: one object per covariance kind
+    for kind in RES_KINDS:
+        obj = os.path.join(OBJDIR, "bv_resident_k%d.o" % kind)
+        jobs.append(([HIPCC] + HIP_FLAGS + ["-DBV_RES_KIND=%d" % kind, "-c",
+                                            os.path.join(CSRC, "bv_resident.hip"), "-o", obj], obj))
     for src in ["bv_prep.cpp"]:
         obj = os.path.join(OBJDIR, src + ".o")
-        _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append((["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
+    _run_parallel([j[0] for j in jobs])
+    objs = [j[1] for j in jobs]
     out = os.path.join(LIBDIR, "libbv%s.so" % tag)
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])

@@ -25,6 +25,7 @@
 #include "bv.h"
 #include "bv_prep.h"
 #include "bv_engine.h"
+#include "bv_device.h"
 
 hipError_t bv_launch_block(int kind, unsigned nblk, size_t lds, hipStream_t st,
                            const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
@@ -32,6 +33,22 @@ hipError_t bv_launch_block_2w(int kind, unsigned nblk, size_t lds, hipStream_t s
                               const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
 hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t st,
                               const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
+template <int KIND>
+hipError_t bv_launch_res_k(int nr, unsigned nblk, hipStream_t st, const bv_block_desc *desc,
+                           const bv_eval_slot *slots, double *ws);
+
+/* The register-resident kernels (bv_resident.hip, one object per kind). */
+static hipError_t bv_launch_res(int kind, int nr, unsigned nblk, hipStream_t st,
+                                const bv_block_desc *desc, const bv_eval_slot *slots,
+                                double *ws) {
+    switch (kind) {
+    case BV_KIND_NU05: return bv_launch_res_k<BV_KIND_NU05>(nr, nblk, st, desc, slots, ws);
+    case BV_KIND_NU15: return bv_launch_res_k<BV_KIND_NU15>(nr, nblk, st, desc, slots, ws);
+    case BV_KIND_NU25: return bv_launch_res_k<BV_KIND_NU25>(nr, nblk, st, desc, slots, ws);
+    case BV_KIND_GENERAL: return bv_launch_res_k<BV_KIND_GENERAL>(nr, nblk, st, desc, slots, ws);
+    default: return bv_launch_res_k<BV_KIND_LOADA>(nr, nblk, st, desc, slots, ws);
+    }
+}
 hipError_t bv_launch_gen(int kind, unsigned max_tiles, unsigned nblk, hipStream_t st,
                          const bv_block_desc *desc, const bv_eval_slot *slots, double *ws);
 hipError_t bv_launch_reduce(unsigned nevals, hipStream_t st, const bv_eval_slot *slots);
@@ -107,7 +124,8 @@ struct bv_ctx {
     int last_k = 0;
     double last_ms = 0.0;
     int iterations = 0;
-    int variant = 0; /* factorization kernel: wavefronts per block 0: four, 1: one, 2: two */
+    int variant = 0; /* factorization kernel: wavefronts per block 0: four, 1: one, 2: two;
+                        3: register-resident (bv_resident.hip) */
     int stepped = 0; /* one launch per panel step over all blocks: 1, or 2 with lookahead;
                         3: one launch per pair of panels with lookahead */
     /* stepped factorization: work items of every step for desc_evals evaluations */
@@ -119,6 +137,7 @@ struct bv_ctx {
     int chunk_split = 1;                    /* chunks: contiguous by cost (1) or round-robin */
     int autotune = 1, tuned = 0;            /* choose the configuration on first use */
     int fused_ok = 1;                       /* every block fits the fused kernels */
+    int res_ok = 0;                         /* ... and the register-resident one */
     hipStream_t cstream[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t cev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int pmax = 0;
@@ -367,6 +386,16 @@ static int bv_side_streams(bv_ctx *c) {
     return BV_OK;
 }
 
+/* one side stream and its fork/join events for the register-resident form */
+static int bv_res_streams(bv_ctx *c) {
+    if (!c->cstream[0] && hipStreamCreateWithFlags(&c->cstream[0], hipStreamNonBlocking) != hipSuccess)
+        return BV_ERR_HIP;
+    for (int i = 0; i < 2; ++i)
+        if (!c->cev[i] && hipEventCreateWithFlags(&c->cev[i], hipEventDisableTiming) != hipSuccess)
+            return BV_ERR_HIP;
+    return BV_OK;
+}
+
 extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     if (!p || !out) return BV_ERR_ARG;
     *out = nullptr;
@@ -395,7 +424,10 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     c->verbose = p->verbose;
     /* factorization kernel: BV_FACTOR_KERNEL=1w selects the one-wavefront one */
     if (const char *v = getenv("BV_FACTOR_KERNEL"))
-        c->variant = (strcmp(v, "1w") == 0) ? 1 : (strcmp(v, "2w") == 0) ? 2 : 0;
+        c->variant = (strcmp(v, "1w") == 0)    ? 1
+                     : (strcmp(v, "2w") == 0)  ? 2
+                     : (strcmp(v, "res") == 0) ? 3
+                                               : 0;
     if (const char *v = getenv("BV_PATH"))
         c->stepped = (strcmp(v, "stepped") == 0)    ? 1
                      : (strcmp(v, "stepped2") == 0) ? 2
@@ -439,6 +471,16 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     /* the fused kernels stage a block's points in LDS (4 Mp doubles): they
        take padded sizes up to BV_MAX_MP; the stepped forms have no limit */
     c->fused_ok = c->max_mp <= BV_MAX_MP;
+    /* the register-resident kernel: padded sizes up to 16 * 24, Euclidean */
+    c->res_ok = c->max_mp <= BV_RES_MAX_MP && c->metric == 0;
+    if (c->variant == 3 && !c->res_ok) {
+        fprintf(stderr,
+                "libbv: the register-resident factorization takes padded sizes up to %d and "
+                "Euclidean distances (largest padded size %d, metric %d)\n",
+                BV_RES_MAX_MP, c->max_mp, c->metric);
+        delete c;
+        return BV_ERR_UNSUPPORTED;
+    }
     if (!c->fused_ok && c->stepped == 0) {
         if (getenv("BV_FACTOR_KERNEL") || getenv("BV_PATH")) {
             fprintf(stderr,
@@ -461,6 +503,10 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
         return BV_ERR_HIP;
     }
     if (c->stepped && c->nstreams > 1 && bv_side_streams(c) != BV_OK) {
+        bv_destroy(c);
+        return BV_ERR_HIP;
+    }
+    if (!c->stepped && c->variant == 3 && bv_res_streams(c) != BV_OK) {
         bv_destroy(c);
         return BV_ERR_HIP;
     }
@@ -495,6 +541,19 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     if (hipStreamSynchronize(c->stream) != hipSuccess) { bv_destroy(c); return BV_ERR_HIP; }
     *out = c;
     return BV_OK;
+}
+
+/* The register-resident kernel over nblk descriptors sorted by padded size,
+   largest first: those above 16 * 20 with the 24-row build, the rest with the
+   20-row build (fewer registers live), on one stream. */
+static hipError_t bv_launch_res_sorted(int kind, const bv_block_desc *h_desc,
+                                       const bv_block_desc *d_desc, unsigned nblk,
+                                       hipStream_t st, const bv_eval_slot *slots, double *ws) {
+    unsigned n6 = 0;
+    while (n6 < nblk && h_desc[n6].mp > BV_RES_MP_SMALL) ++n6;
+    hipError_t e = bv_launch_res(kind, 6, n6, st, d_desc, slots, ws);
+    if (e != hipSuccess) return e;
+    return bv_launch_res(kind, 5, nblk - n6, st, d_desc + n6, slots, ws);
 }
 
 /* The kernels of one call after its slots are on the device: covariance
@@ -559,6 +618,27 @@ static int bv_enqueue(bv_ctx *c, unsigned nblk, int kinds_used) {
             BV_HIP(bv_launch_step_parts((int)nblk, c->stream, c->d_desc, c->d_slots,
                                         c->d_part_logdet, c->d_part_dot, c->d_part_info,
                                         c->pmax));
+        } else if (c->variant == 3) {
+            /* descriptors of k evaluations: each block's k copies in a row,
+               blocks sorted by padded size */
+            const unsigned k = (unsigned)c->desc_evals;
+            unsigned n6 = 0;
+            while (n6 < (unsigned)c->B && c->base_desc[n6].mp > BV_RES_MP_SMALL) ++n6;
+            n6 *= k;
+            if (n6 > 0 && n6 < nblk && c->cstream[0]) {
+                /* the few large blocks on a side stream beside the others */
+                BV_HIP(hipEventRecord(c->cev[1], c->stream));
+                BV_HIP(hipStreamWaitEvent(c->cstream[0], c->cev[1], 0));
+                BV_HIP(bv_launch_res(kind, 6, n6, c->cstream[0], c->d_desc, c->d_slots, c->d_ws));
+                BV_HIP(bv_launch_res(kind, 5, nblk - n6, c->stream, c->d_desc + n6, c->d_slots,
+                                     c->d_ws));
+                BV_HIP(hipEventRecord(c->cev[0], c->cstream[0]));
+                BV_HIP(hipStreamWaitEvent(c->stream, c->cev[0], 0));
+            } else {
+                BV_HIP(bv_launch_res(kind, 6, n6, c->stream, c->d_desc, c->d_slots, c->d_ws));
+                BV_HIP(bv_launch_res(kind, 5, nblk - n6, c->stream, c->d_desc + n6, c->d_slots,
+                                     c->d_ws));
+            }
         } else if (c->variant == 1)
             BV_HIP(bv_launch_block_1w(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));
         else if (c->variant == 2)
@@ -701,7 +781,13 @@ struct bv_cfg {
 static const bv_cfg bv_cfgs[] = {{0, 0, 1, 1}, {1, 0, 1, 1}, {2, 0, 1, 1}, {0, 1, 1, 1},
                                  {0, 1, 2, 2}, {0, 1, 3, 3}, {0, 2, 2, 2}, {0, 2, 3, 3},
                                  {0, 3, 1, 1}, {0, 3, 2, 2}, {0, 3, 3, 3}, {0, 2, 2, 0},
-                                 {0, 3, 2, 0}};
+                                 {0, 3, 2, 0}, {3, 0, 1, 1}};
+
+/* whether a context's blocks fit a configuration's kernels */
+static bool bv_cfg_ok(const bv_ctx *c, const bv_cfg &g) {
+    if (g.stepped) return true;
+    return g.variant == 3 ? c->res_ok != 0 : c->fused_ok != 0;
+}
 
 static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
     /* the step items depend on the chunks and on whether they are skewed */
@@ -713,6 +799,7 @@ static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
     c->nstreams = g.nstreams;
     if (relayout) c->desc_evals = 0; /* the step items depend on the chunks */
     if (c->stepped && c->nstreams > 1) return bv_side_streams(c);
+    if (!c->stepped && c->variant == 3) return bv_res_streams(c);
     return BV_OK;
 }
 
@@ -819,7 +906,7 @@ static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *ou
         const std::string key = bv_tune_key(c, k, kinds_used);
         int idx = -1, graph = 1;
         if (bv_tune_cache_get(path, key, &idx, &graph) && idx >= 0 && idx < ncfg &&
-            (c->fused_ok || bv_cfgs[idx].stepped)) {
+            bv_cfg_ok(c, bv_cfgs[idx])) {
             int rc = bv_apply_cfg(c, bv_cfgs[idx]);
             if (rc) return rc;
             c->tuned = 1;
@@ -829,7 +916,7 @@ static int bv_run(bv_ctx *c, int k, int nparam, const double *thetas, double *ou
         int best = -1, best_fused = -1;
         double best_ms = INFINITY, fused_ms = INFINITY;
         for (int i = 0; i < ncfg; ++i) {
-            if (!c->fused_ok && !bv_cfgs[i].stepped) continue;
+            if (!bv_cfg_ok(c, bv_cfgs[i])) continue;
             int rc = bv_apply_cfg(c, bv_cfgs[i]);
             if (rc) return rc;
             double ms = INFINITY;
@@ -1377,8 +1464,8 @@ static int bv_server_launch(bv_server *s, bv_lane &L) {
     items_bound += (int64_t)(pmax_need + 1) * 16 * 15;
     if ((rc = bv_lane_reserve(L, ws, blocks, k, items_bound, pmax_need))) return rc;
     /* groups: (kind, form), form = the dataset's factorization (0 fused,
-       1..3 stepped forms) */
-    const int NG = 8 * 4;
+       1..3 stepped forms, 4 register-resident) */
+    const int NF = 5, NG = 8 * NF;
     int64_t gcount[NG] = {0};
     int gpmax[NG] = {0};
     std::vector<int> form_of(k);
@@ -1400,8 +1487,8 @@ static int bv_server_launch(bv_server *s, bv_lane &L) {
         sl.result = L.d_result + 4 * e;
         sl.log2pi = log(2 * 3.141592653589793);
         sl.kp.cheb = L.d_cheb + (size_t)e * BV_CHEB_SIZE;
-        form_of[e] = c->stepped;
-        const int g = sl.kind * 4 + c->stepped;
+        form_of[e] = c->stepped ? c->stepped : (c->variant == 3 ? 4 : 0);
+        const int g = sl.kind * NF + form_of[e];
         gcount[g] += c->B;
         gpmax[g] = std::max(gpmax[g], c->max_mp / BV_NB);
         blk_off += c->B;
@@ -1413,7 +1500,7 @@ static int bv_server_launch(bv_server *s, bv_lane &L) {
     int64_t ws_off = 0;
     for (int e = 0; e < k; ++e) {
         const bv_ctx *c = s->datasets[L.reqs[e].dataset];
-        const int g = L.h_slots[e].kind * 4 + form_of[e];
+        const int g = L.h_slots[e].kind * NF + form_of[e];
         for (const bv_block_desc &d0 : c->base_desc) {
             bv_block_desc d = d0;
             d.eval = e;
@@ -1432,8 +1519,8 @@ static int bv_server_launch(bv_server *s, bv_lane &L) {
     int64_t gitem0[NG] = {0};
     int64_t nitems = 0;
     for (int g = 0; g < NG; ++g) {
-        if (!gcount[g] || g % 4 == 0) continue;
-        bv_build_items(L.h_desc + start[g], (size_t)gcount[g], g % 4, 1, false, 1, 1, gpmax[g],
+        if (!gcount[g] || g % NF == 0 || g % NF == 4) continue;
+        bv_build_items(L.h_desc + start[g], (size_t)gcount[g], g % NF, 1, false, 1, 1, gpmax[g],
                        items, goff[g]);
         if (nitems + (int64_t)items.size() > L.cap_items) return BV_ERR_BUSY; /* not sized */
         memcpy(L.h_items + nitems, items.data(), sizeof(bv_step_item) * items.size());
@@ -1450,16 +1537,21 @@ static int bv_server_launch(bv_server *s, bv_lane &L) {
                               hipMemcpyHostToDevice, L.stream));
     const int pst = L.cap_pmax; /* partial-sum stride */
     bool general = false;
-    for (int g = 0; g < NG; ++g) general = general || (gcount[g] && g / 4 == BV_KIND_GENERAL);
+    for (int g = 0; g < NG; ++g) general = general || (gcount[g] && g / NF == BV_KIND_GENERAL);
     if (general) BV_HIP(bv_launch_cheb_build((unsigned)k, L.stream, L.d_slots));
     for (int g = 0; g < NG; ++g) {
         const unsigned nblk = (unsigned)gcount[g];
         if (!nblk) continue;
-        const int kd = g / 4, form = g % 4;
+        const int kd = g / NF, form = g % NF;
         const bv_block_desc *dd = L.d_desc + start[g];
         if (kd == BV_KIND_SPACETIME) {
             const unsigned T = (unsigned)(gmax_mp[g] / BV_TILE);
             BV_HIP(bv_launch_gen(kd, T * (T + 1) / 2, nblk, L.stream, dd, L.d_slots, L.d_ws));
+        }
+        if (form == 4) {
+            BV_HIP(bv_launch_res_sorted(kd, L.h_desc + start[g], dd, nblk, L.stream, L.d_slots,
+                                        L.d_ws));
+            continue;
         }
         if (form == 0) {
             const size_t lds = sizeof(double) * 4 * (size_t)gmax_mp[g];

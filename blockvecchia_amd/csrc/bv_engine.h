@@ -31,6 +31,8 @@
 #define BV_WAVES_PER_SIMD 2 /* occupancy target of the factorization kernel       */
 #endif
 #define BV_MAX_MP 1024    /* largest padded augmented size handled (LDS staging)  */
+#define BV_RES_MAX_MP 384 /* register-resident kernel: padded sizes up to 16 * 24 */
+#define BV_RES_MP_SMALL 320 /* ... its 20-row build takes padded sizes up to this */
 
 /* One block of one evaluation, in launch order. */
 struct bv_block_desc {

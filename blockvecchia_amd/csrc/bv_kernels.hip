@@ -35,8 +35,8 @@
  */
 #include <hip/hip_runtime.h>
 #include "bv_engine.h"
+#include "bv_device.h"
 
-typedef double bv_d4 __attribute__((ext_vector_type(4)));
 
 /* Wavefront trace (builds with BV_TRACE only; see bv_trace_read in bv.h):
    each wavefront of a factorization kernel appends {start, end, id} when it
@@ -71,49 +71,6 @@ struct bv_trace_wave {
 #else
 #define BV_TRACE_WAVE(kernel, step) ((void)0)
 #endif
-
-#define BV_KIND_LOADA 7 /* factor kernel reads A from the workspace tiles */
-
-__device__ __forceinline__ bv_d4 bv_mfma(double a, double b, bv_d4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-/* tile (rt, ct), rt >= ct, of a factor with T tile rows; packed lower
-   triangle, tile-column major; in doubles */
-__device__ __forceinline__ int64_t bv_tile_off(int rt, int ct, int T) {
-    return ((int64_t)(ct * T - (ct * (ct - 1)) / 2 + (rt - ct))) << 8;
-}
-
-/* Entry (p, q) of the augmented matrix of bv_engine.h. */
-template <int KIND>
-__device__ __forceinline__ double bv_aug_entry(int p, int q, int m, double xp, double yp,
-                                               double zp, double xq, double yq, double zq,
-                                               double oq, const bv_kparams &kp) {
-    if (p < m && q < m) return bv_cov<KIND>(xp, yp, zp, xq, yq, zq, kp);
-    if (p == m && q < m) return oq;
-    return (p == q) ? 1.0 : 0.0;
-}
-
-/* Location / observation of augmented index p of block d. */
-__device__ __forceinline__ void bv_point(const bv_block_desc &d, const bv_eval_slot &E, int p,
-                                         double &xv, double &yv, double &zv, double &ov) {
-    xv = yv = zv = ov = 0.0;
-    if (p < d.cs0) {
-        const int64_t o = (int64_t)d.block * E.cs + p;
-        if (E.nbr) {
-            const int64_t idx = E.nbr[o];
-            xv = E.x[idx]; yv = E.y[idx]; ov = E.obs[idx];
-            if (E.z) zv = E.z[idx];
-        } else {
-            xv = E.cx[o]; yv = E.cy[o]; ov = E.cobs[o];
-            if (E.cz) zv = E.cz[o];
-        }
-    } else if (p < d.m) {
-        const int64_t idx = d.obs_off + (p - d.cs0);
-        xv = E.x[idx]; yv = E.y[idx]; ov = E.obs[idx];
-        if (E.z) zv = E.z[idx];
-    }
-}
 
 /* acc[s][ct] -= L(panel rows ct) L(row tile s)^T over the first 2j tile
    columns, for the C row tiles rt0 + wave + 4 (i0 + s) (acc holds A on entry;
@@ -292,30 +249,6 @@ __device__ __forceinline__ void bv_tile_solve(double *__restrict__ ws, int T, in
     }
 }
 
-/* Makes one wavefront's LDS writes visible to its other lanes: a wavefront-
-   scope release/acquire around the wave barrier, so the compiler neither keeps
-   stale LDS values in registers nor moves accesses across it. */
-__device__ __forceinline__ void bv_wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-/* 64-bit readlane: the value of v in lane `src` (wave-uniform). */
-__device__ __forceinline__ double bv_readlane(double v, int src) {
-    const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, src);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), src);
-    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-/* Y = R J^T for 16x16 tiles in accumulator layout (matrix-core products). */
-__device__ __forceinline__ bv_d4 bv_rjt(const bv_d4 &r, const bv_d4 &jm, bv_d4 acc) {
-#pragma unroll
-    for (int g = 0; g < 4; ++g) acc = bv_mfma(jm[g], r[g], acc);
-    return acc;
-}
-
 /* Transpose of a 16x16 tile in accumulator layout, through an LDS scratch of
    16 x 17 doubles (one wavefront). */
 __device__ __forceinline__ bv_d4 bv_transpose16(const bv_d4 &v, int lane, double *sT) {
@@ -346,14 +279,6 @@ __device__ __forceinline__ bv_d4 bv_transpose16(const bv_d4 &v, int lane, double
 #ifndef BV_SWEEP_DPP
 #define BV_SWEEP_DPP 1
 #endif
-
-/* The value of v in lane K of this lane's row of 16 lanes: one 64-bit DPP
-   move (row_newbcast writes every lane, so the destination needs no initial
-   value), no LDS round trip. */
-template <int K>
-__device__ __forceinline__ double bv_row_bcast(double v) {
-    return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xF, 0xF, false);
-}
 
 /* Step K of the sweep below without LDS: the column-K factors of the other
    rows that the update needs are taken from row K (the trailing block is

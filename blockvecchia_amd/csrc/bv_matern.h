@@ -414,6 +414,92 @@ BV_HD double bv_matern_closed(double d, const bv_kparams &kp) {
     return kp.sigma2 * (1.0 + e + e * e * (1.0 / 3.0)) * x;
 }
 
+/* e^-e for e >= 0, Tang's table-driven method (P. T. P. Tang, ACM TOMS 15
+ * (1989) 144) with 64 table points:
+ *   k = rint(e 64 / ln 2),  r = e - k ln2/64 (|r| <= ln2/128; two-term
+ *   reduction with fused multiply-adds),  e^-e = 2^(-k/64) e^-r;
+ *   -k = 64 M + j (0 <= j < 64):  e^-e = 2^M (T_j + (L_j + T_j P(-r))),
+ *   T_j + L_j = 2^(j/64) to twice double precision (T_j rounded to nearest,
+ *   L_j the rounded remainder), P(t) = e^t - 1 = t + t^2/2 + ... + t^5/120
+ *   (truncation below |r|^6 / 720 = 3.5e-17 relative).
+ * Within one ulp of exp(-e) (tests/test_knu_host.py); used for the
+ * closed-form Matern entries of the register-resident factorization, where
+ * the table sits in LDS (tab = T_0..T_63, L_0..L_63). */
+#define BV_EXP_TAB 64
+#define BV_LN2_64_HI 0.010830424696249145     /* ln 2 / 64 rounded         */
+#define BV_LN2_64_LO 3.623510646634843e-19     /* ln 2 / 64 - BV_LN2_64_HI */
+#define BV_64_LN2 92.33248261689366            /* 64 / ln 2                 */
+BV_HD double bv_expneg_tab(double e, const double *tab) {
+    const double kd = rint(e * BV_64_LN2);
+    double r = fma(-kd, BV_LN2_64_HI, e);
+    r = fma(-kd, BV_LN2_64_LO, r);
+    const double t = -r;
+    double q = fma(t, 1.0 / 120.0, 1.0 / 24.0);
+    q = fma(t, q, 1.0 / 6.0);
+    q = fma(t, q, 0.5);
+    const double pm1 = fma(t * t, q, t);
+    const int n = -(int)kd;
+    const double th = tab[n & 63], tl = tab[BV_EXP_TAB + (n & 63)];
+    return ldexp(th + fma(th, pm1, tl), n >> 6);
+}
+
+/* T_j = 2^(j/64) rounded to nearest and L_j = 2^(j/64) - T_j rounded, from
+ * 50-digit values (mpmath: T = float(mp.power(2, mp.mpf(j)/64)),
+ * L = float(mp.power(2, mp.mpf(j)/64) - T)). */
+#define BV_EXP_TAB_HI { \
+    1.0, 1.0108892860517005, 1.0218971486541166, \
+    1.0330248790212284, 1.0442737824274138, 1.0556451783605572, \
+    1.0671404006768237, 1.0787607977571199, 1.0905077326652577, \
+    1.102382583307841, 1.1143867425958924, 1.1265216186082418, \
+    1.1387886347566916, 1.1511892299529827, 1.1637248587775775, \
+    1.1763969916502812, 1.189207115002721, 1.202156731452703, \
+    1.215247359980469, 1.22848053610687, 1.241857812073484, \
+    1.255380757024691, 1.2690509571917332, 1.2828700160787783, \
+    1.2968395546510096, 1.3109612115247644, 1.3252366431597413, \
+    1.339667524053303, 1.3542555469368927, 1.3690024229745905, \
+    1.383909881963832, 1.3989796725383112, 1.4142135623730951, \
+    1.42961333839197, 1.4451808069770467, 1.460917794180647, \
+    1.4768261459394993, 1.4929077282912648, 1.5091644275934228, \
+    1.5255981507445384, 1.5422108254079407, 1.559004400237837, \
+    1.5759808451078865, 1.593142151342267, 1.6104903319492543, \
+    1.6280274218573478, 1.645755478153965, 1.6636765803267364, \
+    1.681792830507429, 1.7001063537185235, 1.718619298122478, \
+    1.7373338352737062, 1.7562521603732995, 1.7753764925265212, \
+    1.7947090750031072, 1.8142521755003989, 1.8340080864093424, \
+    1.8539791250833855, 1.8741676341103, 1.8945759815869656, \
+    1.9152065613971474, 1.9360617934922943, 1.9571441241754002, \
+    1.978456026387951 \
+}
+#define BV_EXP_TAB_LO { \
+    0.0, -1.5234778603368577e-17, 5.109225028973444e-17, \
+    7.600838874027088e-18, 8.551889705537965e-17, 1.759325738772092e-18, \
+    -7.899853966841582e-17, -6.656660436056593e-17, -3.046782079812471e-17, \
+    5.2660368715706944e-17, 1.0410278456845571e-16, 5.165856758795457e-17, \
+    8.912812676025408e-17, 3.250710218863827e-17, 3.8292048369240935e-17, \
+    5.554203254218079e-17, 3.982015231465646e-17, 6.644981499252301e-17, \
+    -7.712630692681488e-17, -1.89878163130253e-17, 4.658027591836937e-17, \
+    -6.7113898212968784e-18, 2.667932131342186e-18, 1.713594918243561e-17, \
+    2.5382502794888315e-17, -7.181536135519454e-17, -2.8587312100388614e-17, \
+    8.927282594831732e-17, 7.70094837980299e-17, 9.593797919118849e-17, \
+    -6.770511658794786e-17, -9.614213209051323e-17, -9.667293313452913e-17, \
+    -1.2031642489053655e-17, -3.0237581349939873e-17, -5.600377186075216e-17, \
+    -3.483994556892796e-17, 1.4192920154284036e-17, -1.016455327754295e-16, \
+    -1.1024941712342561e-16, 7.949834809697621e-17, 3.7812070533575275e-17, \
+    -1.0136916471278304e-17, -1.0094406542311964e-16, 2.4707192569797888e-17, \
+    -6.712955084707084e-17, -1.0125679913674773e-16, 5.8909926967131e-17, \
+    8.199010020581497e-17, -8.0237193703977e-18, -1.851380418263111e-17, \
+    3.164389299292957e-17, 2.960140695448873e-17, 6.429731796556572e-17, \
+    1.8227458427912087e-17, -9.969531538920349e-17, 3.283107224245627e-17, \
+    9.761887490727594e-17, -6.122763413004143e-17, 3.4034035352165297e-17, \
+    -1.0619946056195963e-16, 1.0332385960676326e-16, 8.960767791036668e-17, \
+    4.0388753109278167e-17 \
+}
+BV_HD double bv_exp_tab_entry(int i) {
+    const double hi[BV_EXP_TAB] = BV_EXP_TAB_HI;
+    const double lo[BV_EXP_TAB] = BV_EXP_TAB_LO;
+    return i < BV_EXP_TAB ? hi[i] : lo[i - BV_EXP_TAB];
+}
+
 /* One covariance entry between locations (x1,y1,z1) and (x2,y2,z2). */
 template <int KIND>
 BV_HD double bv_cov(double x1, double y1, double z1, double x2, double y2, double z2,

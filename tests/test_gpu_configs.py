@@ -36,6 +36,9 @@ FORMS = {"fused": {"BV_PATH": "fused", "BV_FACTOR_KERNEL": "4w"},
          "stepped": {"BV_PATH": "stepped", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
          "stepped2": {"BV_PATH": "stepped2", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "0"},
          "paired": {"BV_PATH": "paired", "BV_CHUNKS": "3", "BV_CHUNK_STREAMS": "3"}}
+# the register-resident form takes padded sizes up to 384: the general-nu
+# shape (cs = 150), not the uneven one (cs = 400)
+RES_FORM = {"BV_FACTOR_KERNEL": "res"}
 
 
 def _say(msg):
@@ -46,7 +49,7 @@ def _say(msg):
 def _engine(blocks, form, monkeypatch):
     for k in ("BV_PATH", "BV_FACTOR_KERNEL", "BV_CHUNKS", "BV_CHUNK_STREAMS"):
         monkeypatch.delenv(k, raising=False)
-    for k, v in FORMS[form].items():
+    for k, v in dict(FORMS, res=RES_FORM)[form].items():
         monkeypatch.setenv(k, v)
     return bv.Engine(blocks)
 
@@ -190,7 +193,7 @@ def _c4():
     return _C4
 
 
-@pytest.mark.parametrize("form", sorted(FORMS))
+@pytest.mark.parametrize("form", sorted(FORMS) + ["res"])
 def test_general_nu_shape_vs_oracle(form, monkeypatch):
     c = _c4()
     eng = _engine(c["b"], form, monkeypatch)

@@ -14,6 +14,7 @@ pytestmark = pytest.mark.gpu
 CONFIGS = {
     "fused": {"BV_PATH": "fused"},
     "fused_1w": {"BV_FACTOR_KERNEL": "1w"},
+    "res": {"BV_FACTOR_KERNEL": "res"},
     "stepped": {"BV_PATH": "stepped"},
     "stepped_2chunks": {"BV_PATH": "stepped", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
     "stepped2_2chunks": {"BV_PATH": "stepped2", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},

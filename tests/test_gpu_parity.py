@@ -249,7 +249,7 @@ def test_many_equals_single_and_deterministic(c1_small):
 
 
 @pytest.mark.parametrize("name", ["stepped2_2chunks_skewed", "paired", "paired_3chunks_2streams",
-                                  "paired_3chunks_skewed"])
+                                  "paired_3chunks_skewed", "res"])
 def test_many_equals_single_stepped_forms(c1_small, name, monkeypatch):
     """k evaluations in one call (descriptors of all of them interleaved in
     every step launch, chunks over streams or skewed) give each evaluation's
@@ -297,14 +297,17 @@ def test_llh_n20000_cs300(c2_blocks):
     _check(c2_blocks, [1.5, 0.1, 0.5])
 
 
-@pytest.mark.parametrize("name", ["stepped2", "paired_2chunks_skewed", "paired_2chunks"])
+@pytest.mark.parametrize("name", ["stepped2", "paired_2chunks_skewed", "paired_2chunks", "res"])
 def test_llh_n20000_cs300_configs(c2_blocks, name, monkeypatch):
-    """The two configurations the autotuner picks between for the benchmark
-    workload, fixed by environment, against the oracle at full size."""
+    """The configurations the autotuner picks between for the benchmark
+    workload, fixed by environment, against the oracle at full size (the
+    register-resident form factors its 352-row blocks on a side stream beside
+    the 320-row ones)."""
     env = {"stepped2": {"BV_PATH": "stepped2", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
            "paired_2chunks_skewed": {"BV_PATH": "paired", "BV_CHUNKS": "2",
                                      "BV_CHUNK_STREAMS": "0"},
-           "paired_2chunks": {"BV_PATH": "paired", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"}}
+           "paired_2chunks": {"BV_PATH": "paired", "BV_CHUNKS": "2", "BV_CHUNK_STREAMS": "2"},
+           "res": {"BV_FACTOR_KERNEL": "res"}}
     for k, v in env[name].items():
         monkeypatch.setenv(k, v)
     _say("case: n=20000 bc=1500 cs=300 nu=0.5 %s" % name)
@@ -332,6 +335,7 @@ FACTOR_CONFIGS = {
     "fused_4w": {"BV_PATH": "fused"},
     "fused_1w": {"BV_FACTOR_KERNEL": "1w"},
     "fused_2w": {"BV_FACTOR_KERNEL": "2w"},
+    "res": {"BV_FACTOR_KERNEL": "res"},
     "stepped": {"BV_PATH": "stepped", "BV_CHUNKS": "1", "BV_CHUNK_STREAMS": "1"},
     "stepped_3chunks_2streams": {"BV_PATH": "stepped", "BV_CHUNKS": "3", "BV_CHUNK_STREAMS": "2"},
     "stepped2": {"BV_PATH": "stepped2", "BV_CHUNKS": "1", "BV_CHUNK_STREAMS": "1"},
@@ -353,7 +357,7 @@ def _candidate_env(cfg):
     env = {"BV_PATH": ["fused", "stepped", "stepped2", "paired"][stepped],
            "BV_CHUNKS": str(chunks), "BV_CHUNK_STREAMS": str(streams)}
     if stepped == 0:
-        env["BV_FACTOR_KERNEL"] = ["4w", "1w", "2w"][variant]
+        env["BV_FACTOR_KERNEL"] = ["4w", "1w", "2w", "res"][variant]
     return env
 
 

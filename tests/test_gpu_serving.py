@@ -169,7 +169,8 @@ def _serve(engines, reqs, lanes, ws_bytes, max_evals):
 
 
 @pytest.mark.parametrize("forms", [("stepped", "stepped2", "paired", "fused"),
-                                   ("paired", "paired", "stepped2", "stepped")])
+                                   ("paired", "paired", "stepped2", "stepped"),
+                                   ("res", "fused", "res", "paired")])
 def test_server_merges_stepped_forms(monkeypatch, forms):
     """Datasets fixed to each factorization form, below the own-path size:
     their requests merge into shared batches (one group of step launches per
@@ -179,8 +180,13 @@ def test_server_merges_stepped_forms(monkeypatch, forms):
     blocks = _datasets()
     engines = []
     for b, form in zip(blocks, forms):
-        monkeypatch.setenv("BV_PATH", form)
-        monkeypatch.setenv("BV_CHUNKS", "2" if form != "fused" else "1")
+        if form == "res":  # the register-resident kernel, a one-launch form
+            monkeypatch.setenv("BV_PATH", "fused")
+            monkeypatch.setenv("BV_FACTOR_KERNEL", "res")
+        else:
+            monkeypatch.setenv("BV_PATH", form)
+            monkeypatch.delenv("BV_FACTOR_KERNEL", raising=False)
+        monkeypatch.setenv("BV_CHUNKS", "2" if form not in ("fused", "res") else "1")
         monkeypatch.setenv("BV_CHUNK_STREAMS", "0")
         engines.append(bv.Engine(b))
     thetas = [[1.5, 0.1, 0.5], [1.0, 0.08, 1.5], [1.2, 0.1, 0.8], [1.4, 0.09, 0.5]]

@@ -33,6 +33,15 @@ extern "C" void shim_cheb_table(double nu, double *tab) {
 extern "C" double shim_xnu_knu_cheb(double nu, double x, const double *tab) {
     return bv_xnu_knu_cheb(x, tab, nu);
 }
+extern "C" double shim_exp_tab_entry(int i) { return bv_exp_tab_entry(i); }
+extern "C" void shim_expneg_tab(const double *e, double *out, long n) {
+    double tab[2 * BV_EXP_TAB];
+    for (int i = 0; i < 2 * BV_EXP_TAB; ++i) tab[i] = bv_exp_tab_entry(i);
+    for (long i = 0; i < n; ++i) out[i] = bv_expneg_tab(e[i], tab);
+}
+extern "C" void shim_expl(const double *e, long double *out, long n) {
+    for (long i = 0; i < n; ++i) out[i] = expl(-(long double)e[i]);
+}
 extern "C" double shim_cov(int kind, double x1, double y1, double x2, double y2, double s2,
                            double beta, double nu, double con) {
     bv_kparams kp = {};
@@ -77,6 +86,12 @@ def shim(tmp_path_factory):
     lib.shim_cheb_table.restype = None
     lib.shim_xnu_knu_cheb.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
     lib.shim_xnu_knu_cheb.restype = ctypes.c_double
+    lib.shim_exp_tab_entry.argtypes = [ctypes.c_int]
+    lib.shim_exp_tab_entry.restype = ctypes.c_double
+    lib.shim_expneg_tab.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    lib.shim_expneg_tab.restype = None
+    lib.shim_expl.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    lib.shim_expl.restype = None
     lib.shim_cov.argtypes = [ctypes.c_int] + [ctypes.c_double] * 8
     lib.shim_cov.restype = ctypes.c_double
     return lib
@@ -201,3 +216,44 @@ def test_chebyshev_knu_half_integer_closed_forms(shim):
                 continue
             v = shim.shim_xnu_knu_cheb(nu, x, tab.ctypes.data)
             assert abs(v - ref) <= 1e-14 * ref, (nu, x, v, ref)
+
+
+def test_exp_table_is_2_pow_j_over_64(shim):
+    """The table of the register-resident kernel's exponential: T_j is
+    2^(j/64) rounded to nearest and T_j + L_j equals 2^(j/64) to ~2^-106,
+    against 40-digit mpmath values."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    for j in range(64):
+        v = mp.power(2, mp.mpf(j) / 64)
+        hi, lo = shim.shim_exp_tab_entry(j), shim.shim_exp_tab_entry(64 + j)
+        assert hi == float(v)
+        assert abs((mp.mpf(hi) + mp.mpf(lo) - v) / v) < mp.mpf(2) ** -104
+
+
+def test_expneg_tab_within_one_ulp(shim):
+    """e^-e by the table-driven method against the host's extended-precision
+    expl over [0, 745] (closed-form Matern arguments d / range), sampled
+    densely near 0 and across the whole range; exact at 0, 0 beyond the
+    smallest subnormal."""
+    rng = np.random.default_rng(0)
+    e = np.concatenate([np.linspace(0.0, 1.0, 200001), rng.uniform(0.0, 745.0, 400000),
+                        10.0 ** rng.uniform(-12, 0, 100000)])
+    n = e.size
+    got = np.empty(n)
+    ref = np.empty(n, dtype=np.longdouble)
+    shim.shim_expneg_tab(e.ctypes.data_as(ctypes.c_void_p), got.ctypes.data_as(ctypes.c_void_p), n)
+    shim.shim_expl(e.ctypes.data_as(ctypes.c_void_p), ref.ctypes.data_as(ctypes.c_void_p), n)
+    normal = ref > np.longdouble(2.3e-308)
+    r = ref[normal].astype(np.float64)
+    ulp = np.spacing(r)
+    err = np.abs((got[normal].astype(np.longdouble) - ref[normal]) / ulp.astype(np.longdouble))
+    assert float(err.max()) < 1.0, float(err.max())
+    one = np.empty(1)
+    shim.shim_expneg_tab(np.zeros(1).ctypes.data_as(ctypes.c_void_p),
+                         one.ctypes.data_as(ctypes.c_void_p), 1)
+    assert one[0] == 1.0
+    big = np.array([746.0, 800.0, 1e4])
+    out = np.empty(3)
+    shim.shim_expneg_tab(big.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), 3)
+    assert np.all(out == 0.0)
