@@ -523,21 +523,40 @@ def main():
     traffic = traffic_from_profiles(cfg) if args.nu == 0.5 else None
     nlaunch = {0: 1, 1: eng_pmax(blocks, cs), 2: eng_pmax(blocks, cs),
                3: (eng_pmax(blocks, cs) + 1) // 2}[cfg[1] if cfg else 0]
+    if cfg and cfg[1] == 0 and cfg[0] == 3:
+        nlaunch = 2  # the register-resident kernel's two builds, on two streams
     if cfg and cfg[1] and cfg[2] > 1 and cfg[3] != 0:
         nlaunch *= cfg[2]
     limiter = None
+    pipe = None
+    try:
+        pipe = bv.fp64_pipe_probe(0)
+        log("fp64 pipe probe (cycles per repetition): %s" %
+            ", ".join("%s %.1f" % kv for kv in pipe.items()))
+    except Exception as e:  # report and keep the main line
+        log("fp64 pipe probe failed: %s" % e)
     if traffic:
         hbm_rate = traffic / (kern_ms * 1e-3) / 1e9
-        limiter = {
-            "hbm_bytes_per_eval": traffic, "hbm_GBps": hbm_rate,
-            "hbm_frac": hbm_rate / HBM_PEAK_GBS,
-            "reading": "algorithmically MFMA-bound (%.1f GFLOP on 2.3 MB of inputs), measured "
+        if cfg and cfg[1] == 0 and cfg[0] == 3:
+            reading = ("the factor stays on chip (register-resident form): %.2f GB per evaluation "
+                       "below L2 (%.0f%% of HBM peak), the matrix cores at %.0f%%; what bounds it is "
+                       "its one wavefront per SIMD (the registers hold the factor): a wavefront's "
+                       "FP64 vector arithmetic (covariance generation, diagonal sweeps) does not "
+                       "overlap its own matrix-core products, only another wavefront's "
+                       "(fp64_pipe_probe), and the 16-step diagonal sweep per tile column is a "
+                       "serial chain (profiles/bv_res_kernel_n20000_cs300.md)"
+                       % (traffic / 1e9, 100 * hbm_rate / HBM_PEAK_GBS,
+                          100 * achieved / FP64_PEAK_TFLOPS))
+        else:
+            reading = ("algorithmically MFMA-bound (%.1f GFLOP on 2.3 MB of inputs), measured "
                        "memory-bound: the factor tiles make %.2f GB of HBM round trips per "
                        "evaluation at %.0f%% of HBM peak while the matrix cores run at %.0f%%; "
                        "waves wait on memory results for 42-49%% of their cycles "
                        "(profiles/factorization_counters_n20000_cs300.md)"
                        % (F / 1e9, traffic / 1e9, 100 * hbm_rate / HBM_PEAK_GBS,
-                          100 * achieved / FP64_PEAK_TFLOPS)}
+                          100 * achieved / FP64_PEAK_TFLOPS))
+        limiter = {"hbm_bytes_per_eval": traffic, "hbm_GBps": hbm_rate,
+                   "hbm_frac": hbm_rate / HBM_PEAK_GBS, "reading": reading}
     out = {
         "metric": METRIC,
         "value": evals_per_s,
@@ -570,6 +589,7 @@ def main():
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
             "measured_limiter": limiter,
+            "fp64_pipe_probe": pipe,
             "kernel": kname,
             "config": cfg,
             "flops_per_eval": F,

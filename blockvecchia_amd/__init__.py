@@ -80,7 +80,8 @@ class BvBlocks(ctypes.Structure):
 EXPORTED = [
     "bv_create", "bv_llh", "llh_Xvecchia_batch", "bv_llh_many", "bv_block_terms",
     "bv_last_status", "bv_last_kernel_ms", "bv_iterations", "bv_destroy", "bv_matern",
-    "bv_partition", "bv_blocks_free", "bv_device_selftest", "bv_trace_reset", "bv_trace_read",
+    "bv_partition", "bv_blocks_free", "bv_device_selftest", "bv_fp64_pipe_probe",
+    "bv_trace_reset", "bv_trace_read",
     "bv_server_create", "bv_server_attach", "bv_server_submit", "bv_server_poll",
     "bv_server_pending", "bv_server_destroy", "bv_factor_config", "bv_set_graph",
     "bv_partition_gpu", "bv_factor_candidates", "bv_llh_noalloc", "bv_simulate",
@@ -130,6 +131,8 @@ def load_library(path=None):
     lib.bv_blocks_free.restype = None
     lib.bv_device_selftest.argtypes = [ctypes.c_int]
     lib.bv_device_selftest.restype = ctypes.c_int
+    lib.bv_fp64_pipe_probe.argtypes = [ctypes.c_int, _dp]
+    lib.bv_fp64_pipe_probe.restype = ctypes.c_int
     lib.bv_factor_config.argtypes = [ctypes.c_void_p, _ip]
     lib.bv_factor_config.restype = ctypes.c_int
     lib.bv_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -417,6 +420,19 @@ def matern(x1, y1, x2, y2, theta, kernel=1, metric=0, dist_scale=1.0, z1=None, z
 
 def device_selftest(device=0):
     return load_library().bv_device_selftest(device)
+
+
+def fp64_pipe_probe(device=0):
+    """Median cycles per loop repetition of the device's FP64 pipe probe
+    (bv_fp64_pipe_probe in include/bv.h): 2 matrix-core products, 8 FP64
+    multiply-adds, both in one wavefront, 2 products + 32 multiply-adds, and
+    the products / multiply-adds wavefronts of the split body."""
+    c = np.zeros(6)
+    rc = load_library().bv_fp64_pipe_probe(device, _d(c))
+    if rc != BV_OK:
+        raise RuntimeError("bv_fp64_pipe_probe failed: %d" % rc)
+    keys = ["mfma2", "fma8", "mfma2_fma8", "mfma2_fma32", "split_mfma2", "split_fma8"]
+    return dict(zip(keys, c.tolist()))
 
 
 def factor_candidates():

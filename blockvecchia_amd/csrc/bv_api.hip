@@ -72,6 +72,8 @@ hipError_t bv_launch_matern(int m, int n, const double *x1, const double *y1, co
                             const double *x2, const double *y2, const double *z2,
                             const bv_kparams &kp, int kind, double *A);
 hipError_t bv_launch_selftest(const double *a, const double *b, double *d);
+hipError_t bv_launch_pipe_probe(int mode, int ncu, int iters, double *out,
+                                unsigned long long *cyc);
 hipError_t bv_launch_sim_finish(unsigned nblk, hipStream_t st, const bv_block_desc *desc,
                                 const bv_eval_slot *slots, const double *ws, const double *z);
 hipError_t bv_launch_kmeans_assign(int64_t n, const double *px, const double *py, const double *pz,
@@ -1139,6 +1141,45 @@ extern "C" int bv_device_selftest(int device) {
             if (fabs(ref - hd[4 * l + g]) > 1e-12 * (1.0 + fabs(ref))) return 1;
         }
     return BV_OK;
+}
+
+extern "C" int bv_fp64_pipe_probe(int device, double *cycles) {
+    if (!cycles) return BV_ERR_ARG;
+    BV_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    BV_HIP(hipGetDeviceProperties(&prop, device));
+    const int ncu = prop.multiProcessorCount, iters = 125; /* 2000 steps */
+    double *out = nullptr;
+    unsigned long long *cyc = nullptr;
+    BV_HIP(hipMalloc(&out, sizeof(double) * 512 * (size_t)ncu));
+    if (hipMalloc(&cyc, sizeof(unsigned long long) * 8 * (size_t)ncu) != hipSuccess) {
+        (void)hipFree(out);
+        return BV_ERR_HIP;
+    }
+    std::vector<unsigned long long> h(8 * (size_t)ncu);
+    int rc = BV_OK;
+    for (int mode = 0; mode < 5 && rc == BV_OK; ++mode) {
+        /* one warm-up launch, then the timed one */
+        if (bv_launch_pipe_probe(mode, ncu, 10, out, cyc) != hipSuccess ||
+            bv_launch_pipe_probe(mode, ncu, iters, out, cyc) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(h.data(), cyc, sizeof(unsigned long long) * h.size(),
+                      hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = BV_ERR_HIP;
+            break;
+        }
+        /* median over the compute units of wavefront 0 (and, split, of
+           wavefront 1: the multiply-add one) */
+        for (int w = 0; w < (mode == 4 ? 2 : 1); ++w) {
+            std::vector<unsigned long long> v;
+            for (int c = 0; c < ncu; ++c) v.push_back(h[8 * (size_t)c + w]);
+            std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+            cycles[mode + w] = (double)v[v.size() / 2] / (16.0 * iters);
+        }
+    }
+    (void)hipFree(out);
+    (void)hipFree(cyc);
+    return rc;
 }
 
 /* ======================================================================

@@ -2084,6 +2084,67 @@ bv_selftest_kernel(const double *a, const double *b, double *d) {
     *(bv_d4 *)(d + 4 * lane) = acc;
 }
 
+/* FP64 pipe probe: each wavefront repeats `iters` times a body of 16 steps,
+   each step NM independent 16x16x4 FP64 matrix-core products (two
+   accumulators) and NV independent FP64 fused multiply-adds (eight
+   registers), and stamps its cycle count.  With `split`, the even
+   wavefronts of a workgroup run only the products and the odd ones only the
+   multiply-adds (two wavefronts per SIMD), so the two kinds come from
+   different wavefronts.  Results are per step. */
+template <int NM, int NV>
+__device__ __forceinline__ void bv_pipe_body(bv_d4 &a0, bv_d4 &a1, double (&v)[8], double x,
+                                             double y, int iters) {
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+#pragma unroll
+            for (int t = 0; t < NM; t += 2) {
+                a0 = bv_mfma(x, y, a0);
+                a1 = bv_mfma(y, x, a1);
+            }
+#pragma unroll
+            for (int r = 0; r < NV; ++r) v[r & 7] = __builtin_fma(v[r & 7], y, x);
+        }
+    }
+}
+
+template <int NM, int NV>
+__global__ void __launch_bounds__(512)
+bv_pipe_probe_kernel(double *out, unsigned long long *cyc, int iters, int split) {
+    const int wave = threadIdx.x >> 6;
+    bv_d4 a0 = (bv_d4){0.0, 0.0, 0.0, 0.0}, a1 = a0;
+    const double x = threadIdx.x * 1e-3, y = 1.0 + threadIdx.x * 1e-4;
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = x + j;
+    const unsigned long long t0 = __builtin_readcyclecounter();
+    if (!split) bv_pipe_body<NM, NV>(a0, a1, v, x, y, iters);
+    else if (!(wave & 1)) bv_pipe_body<NM, 0>(a0, a1, v, x, y, iters);
+    else bv_pipe_body<0, NV>(a0, a1, v, x, y, iters);
+    const unsigned long long t1 = __builtin_readcyclecounter();
+    double t = a0[0] + a1[1];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t += v[j];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = t;
+    if ((threadIdx.x & 63) == 0) cyc[blockIdx.x * 8 + wave] = t1 - t0;
+}
+
+/* mode 0: 2 products; 1: 8 multiply-adds; 2: both; 3: 2 products and 32
+   multiply-adds; 4: mode 2 split over two wavefronts per SIMD.  One
+   workgroup per compute unit (ncu of them), 256 threads (512 split). */
+hipError_t bv_launch_pipe_probe(int mode, int ncu, int iters, double *out,
+                                unsigned long long *cyc) {
+    const dim3 g(ncu), b(mode == 4 ? 512 : 256);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((bv_pipe_probe_kernel<2, 0>), g, b, 0, 0, out, cyc, iters, 0); break;
+    case 1: hipLaunchKernelGGL((bv_pipe_probe_kernel<0, 8>), g, b, 0, 0, out, cyc, iters, 0); break;
+    case 2: hipLaunchKernelGGL((bv_pipe_probe_kernel<2, 8>), g, b, 0, 0, out, cyc, iters, 0); break;
+    case 3: hipLaunchKernelGGL((bv_pipe_probe_kernel<2, 32>), g, b, 0, 0, out, cyc, iters, 0); break;
+    default: hipLaunchKernelGGL((bv_pipe_probe_kernel<2, 8>), g, b, 0, 0, out, cyc, iters, 1); break;
+    }
+    return hipGetLastError();
+}
+
 /* ---- host-side launch wrappers (called from bv_api.hip) ---- */
 hipError_t bv_launch_block_1w(int kind, unsigned nblk, size_t lds, hipStream_t st,
                               const bv_block_desc *desc, const bv_eval_slot *slots, double *ws) {

@@ -8,61 +8,36 @@
  * rows r = w + 4 i (i < NR).  Every factor tile L(r, k) stays in that
  * wavefront's registers from its solve (column k) until the column of its own
  * diagonal, the last one that reads it; nothing of the factor is written to
- * memory.  For column c:
- *   - the finished tiles of pivot row c, L(c, 0..c-1), are in an LDS buffer,
- *     negated, copied there by their owner;
- *   - C(r, c) = A(r, c) - sum_k L(r, k) L(c, k)^T for the own rows r >= c:
- *     FP64 matrix-core products with the own tile as one operand and the LDS
- *     tile as the other, A generated in registers from the staged points;
+ * memory.  Column c (a compile-time constant: the column loop is unrolled, so
+ * every register index is static):
+ *   - C(r, c) = A(r, c) - sum_k L(r, k) L(c, k)^T for the own rows r >= c,
+ *     FP64 matrix-core products with the own tile as one operand and pivot
+ *     row c's tile (LDS, negated) as the other;
  *   - the owner of row c factors the 16x16 diagonal C(c, c) in registers
- *     (L_cc, W_c = L_cc^-1, pivots, row m's squares) and publishes W_c;
- *   - L(r, c) = C(r, c) W_c^T for the own rows below.
- * Lookahead: while column c's diagonal is factored, C(r, c+1) over k < c is
- * already formed, in the same instruction stream, so the matrix-core products
- * issue in the gaps of the sweep's dependent chain.  The rows c+1 and c+2 are
- * solved first after W_c is published (their owners copy them to the LDS
- * buffers the next two columns read); the other rows' solves of column c run
- * at the start of column c+1.  Per column: one sweep, two barriers.
+ *     (L_cc, W_c = L_cc^-1, pivots, row m's squares) and publishes W_c in
+ *     LDS; its matrix-core jobs of the column run between the sweep's steps;
+ *   - L(r, c) = C(r, c) W_c^T for the own rows below (rows c+1 and c+2 right
+ *     after W_c, so their owners can publish them; the others at the start of
+ *     column c+1, where they also give C(r, c+1) its last term).
+ * Lookahead: C(r, c+1) over k < c is formed during column c (seeded with
+ * A(r, c+1) from LDS), so each column costs one diagonal sweep, the last
+ * term of the next pivot block and two barriers, the products filling the
+ * rest.  The wavefronts that do not own column c's diagonal generate the
+ * covariance tiles of column c+2 into LDS.
  *
- * LDS: the staged points (4 x 64 NR doubles), three pivot-row buffers of
- * 4 NR - 1 tiles (rows c, c+1, c+2 of the schedule), two W slots.  With
- * NR = 6 that is 154 KB: one workgroup per compute unit, four wavefronts of up
- * to 512 registers each, one per SIMD.
- *
- * All wavefronts run the same instructions: the sweep is computed by every
- * wavefront on its tile row q's accumulator (the owner's is the pivot block,
- * the others discard theirs), so the column has no branch around it and the
- * scheduler can interleave it with the products.  Rows that a wavefront holds
- * only statically (finished, or beyond the block) carry values nobody reads.
+ * LDS: two slots of 4 NR tiles (slot j % 2: pivot row j's tiles at t < j,
+ * column j's covariance tiles at t >= j), two W slots, the staged points
+ * (4 x 64 NR doubles), the exponential's table: 97 KB (NR = 5) or 113 KB
+ * (NR = 6).  One workgroup per compute unit, four wavefronts of up to 512
+ * registers each, one per SIMD.  Rows a wavefront holds only statically
+ * (finished, or beyond the block) are skipped by wave-uniform branches or
+ * carry values nobody reads.
  */
 #include <hip/hip_runtime.h>
 #include "bv_engine.h"
 #include "bv_device.h"
 
 #define BV_RES_THREADS 256
-#ifndef BV_RES_VALU_PER_MFMA
-#define BV_RES_VALU_PER_MFMA 4 /* issue pattern of the job sections */
-#endif
-
-/* Timing build (-DBV_RES_TIMING, development only): every wavefront of the
-   first BV_RES_TBLK workgroups stamps the shader clock at five points of each
-   column; bv_res_timing_read copies the stamps out. */
-#ifdef BV_RES_TIMING
-#define BV_RES_TBLK 8
-#define BV_RES_TCOL 24
-static __device__ unsigned long long bv_res_tstamp[BV_RES_TBLK][4][BV_RES_TCOL][5];
-#ifndef BV_RES_TMASK
-#define BV_RES_TMASK 31
-#endif
-#define BV_RES_STAMP(C, P)                                                                  \
-    do {                                                                                    \
-        if (((BV_RES_TMASK) >> (P)) & 1)                                                    \
-            if (blockIdx.x < BV_RES_TBLK && (threadIdx.x & 63) == 0)                       \
-                bv_res_tstamp[blockIdx.x][threadIdx.x >> 6][C][P] = __builtin_readcyclecounter(); \
-    } while (0)
-#else
-#define BV_RES_STAMP(C, P) ((void)0)
-#endif
 
 namespace {
 
@@ -79,7 +54,6 @@ struct bv_res_state {
 
 struct bv_res_block {
     int T, NC, m, cs0, w, lane;
-    bool planar;                      /* closed-form kinds in the plane          */
     const double *sx, *sy, *sz, *so;  /* staged points of the block (LDS)        */
     const double *etab;               /* 2^(j/64) as hi, lo pairs (LDS)          */
     double *buf;                      /* two slots of pivot-row / column tiles   */
@@ -344,7 +318,6 @@ __device__ __forceinline__ void bv_res_col(bv_res_state<NR> &S, const bv_res_blo
         constexpr int q = C / 4, s = C % 4;
         const int lane = B.lane, w = B.w;
         const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
-        BV_RES_STAMP(C, 0);
 
         /* row q first: it holds the pivot block on the owner (its solve of
            column C-1 was done at the end of column C-1) */
@@ -355,7 +328,6 @@ __device__ __forceinline__ void bv_res_col(bv_res_state<NR> &S, const bv_res_blo
         } else {
             S.acc[q] = S.nxt[q];
         }
-        BV_RES_STAMP(C, 1);
         /* nothing moves across: the pivot block's accumulation ends here */
         __builtin_amdgcn_sched_barrier(0);
 
@@ -389,9 +361,7 @@ __device__ __forceinline__ void bv_res_col(bv_res_state<NR> &S, const bv_res_blo
                 bv_res_gen_col<KIND>(B, kp, C + 2, C + 2 + rank, 3);
             }
         }
-        BV_RES_STAMP(C, 2);
         __syncthreads();
-        BV_RES_STAMP(C, 3);
 
         /* rows C+1 and C+2 of column C: their owners copy them to the
            buffers of the next two pivot rows (row C+2 with its older tiles) */
@@ -419,7 +389,6 @@ __device__ __forceinline__ void bv_res_col(bv_res_state<NR> &S, const bv_res_blo
                 }
             }
         }
-        BV_RES_STAMP(C, 4);
         __syncthreads();
         bv_res_col<KIND, NR, C + 1>(S, B, kp);
     }
@@ -453,7 +422,6 @@ bv_res_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__rest
     B.cs0 = d.cs0;
     B.w = __builtin_amdgcn_readfirstlane(tid >> 6);
     B.lane = tid & 63;
-    B.planar = E.kp.metric == 0 && !E.kp.use_z;
     B.sx = s_x; B.sy = s_y; B.sz = s_z; B.so = s_o;
     B.etab = s_etab;
     B.buf = s_buf;
@@ -548,11 +516,3 @@ hipError_t bv_launch_res_k<BV_RES_KIND>(int nr, unsigned nblk, hipStream_t st,
         hipLaunchKernelGGL((bv_res_kernel<BV_RES_KIND, 6>), grid, blk, 0, st, desc, slots, ws);
     return hipGetLastError();
 }
-
-#if defined(BV_RES_TIMING) && BV_RES_KIND == 0
-extern "C" int bv_res_timing_read(unsigned long long *dst) {
-    return hipMemcpyFromSymbol(dst, HIP_SYMBOL(bv_res_tstamp), sizeof(bv_res_tstamp)) == hipSuccess
-               ? 0
-               : -1;
-}
-#endif

@@ -299,6 +299,21 @@ void bv_blocks_free(bv_blocks *b);
  * current device (one 16x16x4 FP64 product against a host product). */
 int bv_device_selftest(int device);
 
+/* FP64 pipe probe of the device (measurement of the kernels' limiter; no
+ * counterpart in the reference).  One workgroup per compute unit; each
+ * wavefront times 2000 repetitions of a step; cycles[] receives the median
+ * cycles per step:
+ *   [0] 2 independent 16x16x4 FP64 matrix-core products,
+ *   [1] 8 independent FP64 fused multiply-adds (64 lanes),
+ *   [2] both in one wavefront,
+ *   [3] 2 products and 32 multiply-adds in one wavefront,
+ *   [4], [5] body [2] split over two wavefronts of one SIMD: the product
+ *            wavefront, the multiply-add wavefront.
+ * Measured on MI355X: [0] 128, [1] 35, [2] 169, [3] 289, [4] 128, [5] 35
+ * cycles: within one wavefront the multiply-adds add to the products' time,
+ * from two wavefronts of one SIMD they overlap completely. */
+int bv_fp64_pipe_probe(int device, double *cycles);
+
 /* Wavefront trace of the factorization kernels, for launch-occupancy
  * analysis.  In a library built with BV_TRACE (`python -m
  * blockvecchia_amd.build --trace` writes lib/libbv_trace.so), every wavefront
