@@ -87,4 +87,41 @@ __device__ __forceinline__ double bv_row_bcast(double v) {
     return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xF, 0xF, false);
 }
 
+/* sqrt(s), s >= 0: the hardware reciprocal square root and the Newton
+   corrections of the library square root, without its rescaling of inputs
+   below 2^-767 (squared distances of distinct points are far above) */
+__device__ __forceinline__ double bv_sqrt_nr(double s) {
+    const double y = __builtin_amdgcn_rsq(s);
+    double g = s * y, h = 0.5 * y;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, s);
+    g = fma(d, h, g);
+    d = fma(-g, g, s);
+    g = fma(d, h, g);
+    return (s == 0.0) ? 0.0 : g;
+}
+
+/* the exponential's table (bv_expneg_tab in bv_matern.h), one copy per code
+   object in device memory */
+static __device__ const double bv_exp_tab_dev[2 * BV_EXP_TAB] = {
+    BV_EXP_TAB_HI_LIST, BV_EXP_TAB_LO_LIST};
+
+/* Closed-form Matern entry (nu = 1/2, 3/2, 5/2) between two points of the
+   plane: the distance as bv_dist_euclid rounds it, the square root and the
+   exponential above (within one ulp each), the forms of bv_matern_closed. */
+template <int KIND>
+__device__ __forceinline__ double bv_matern_planar(double xa, double ya, double xb, double yb,
+                                                   const bv_kparams &kp) {
+#pragma clang fp contract(off)
+    const double dx = xb - xa, dy = yb - ya;
+    const double d = bv_sqrt_nr(dx * dx + dy * dy);
+    const double t = d * kp.inv_scale;
+    const double x = bv_expneg_tab(t, bv_exp_tab_dev);
+    if (KIND == BV_KIND_NU05) return kp.sigma2 * x;
+    if (KIND == BV_KIND_NU15) return kp.sigma2 * (1.0 + t) * x;
+    return kp.sigma2 * (1.0 + t + t * t * (1.0 / 3.0)) * x;
+}
+
 #endif /* BV_DEVICE_H */

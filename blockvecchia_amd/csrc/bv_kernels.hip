@@ -156,7 +156,7 @@ __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
                     const int q = q0 + 4 * g + lq;
-                    v[g] = bv_matern_closed<KIND>(bv_dist_euclid(xp, yp, 0.0, sx[q], sy[q], 0.0, 0), kp);
+                    v[g] = bv_matern_planar<KIND>(xp, yp, sx[q], sy[q], kp);
                 }
             } else {
 #pragma unroll
@@ -909,8 +909,7 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems,
 #pragma unroll
                         for (int g = 0; g < 4; ++g) {
                             const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
-                            v[g] = bv_matern_closed<KIND>(
-                                bv_dist_euclid(qx[ip], qy[ip], 0.0, qx[iq], qy[iq], 0.0, 0), kp);
+                            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp);
                         }
                     } else {
 #pragma unroll
@@ -1112,8 +1111,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
 #pragma unroll
                         for (int g = 0; g < 4; ++g) {
                             const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
-                            v[g] = bv_matern_closed<KIND>(
-                                bv_dist_euclid(qx[ip], qy[ip], 0.0, qx[iq], qy[iq], 0.0, 0), kp);
+                            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp);
                         }
                     } else {
 #pragma unroll
@@ -1197,8 +1195,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
                         const int iq = ct * BV_TILE + 4 * g + lq; /* own rows are the columns */
-                        v[g] = bv_matern_closed<KIND>(
-                            bv_dist_euclid(qx[ip], qy[ip], 0.0, qx[iq], qy[iq], 0.0, 0), kp);
+                        v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp);
                     }
                 } else {
 #pragma unroll
@@ -1365,8 +1362,7 @@ __device__ __forceinline__ bv_d4 bv_pair_tile(const double *__restrict__ ws, int
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int iq = iq0 + 4 * g + lq;
-            v[g] = bv_matern_closed<KIND>(bv_dist_euclid(qx[ip], qy[ip], 0.0, qx[iq], qy[iq], 0.0, 0),
-                                          kp);
+            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp);
         }
     } else {
 #pragma unroll

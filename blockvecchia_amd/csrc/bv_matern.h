@@ -446,7 +446,9 @@ BV_HD double bv_expneg_tab(double e, const double *tab) {
 /* T_j = 2^(j/64) rounded to nearest and L_j = 2^(j/64) - T_j rounded, from
  * 50-digit values (mpmath: T = float(mp.power(2, mp.mpf(j)/64)),
  * L = float(mp.power(2, mp.mpf(j)/64) - T)). */
-#define BV_EXP_TAB_HI { \
+#define BV_EXP_TAB_HI { BV_EXP_TAB_HI_LIST }
+#define BV_EXP_TAB_LO { BV_EXP_TAB_LO_LIST }
+#define BV_EXP_TAB_HI_LIST \
     1.0, 1.0108892860517005, 1.0218971486541166, \
     1.0330248790212284, 1.0442737824274138, 1.0556451783605572, \
     1.0671404006768237, 1.0787607977571199, 1.0905077326652577, \
@@ -468,9 +470,8 @@ BV_HD double bv_expneg_tab(double e, const double *tab) {
     1.7947090750031072, 1.8142521755003989, 1.8340080864093424, \
     1.8539791250833855, 1.8741676341103, 1.8945759815869656, \
     1.9152065613971474, 1.9360617934922943, 1.9571441241754002, \
-    1.978456026387951 \
-}
-#define BV_EXP_TAB_LO { \
+    1.978456026387951
+#define BV_EXP_TAB_LO_LIST \
     0.0, -1.5234778603368577e-17, 5.109225028973444e-17, \
     7.600838874027088e-18, 8.551889705537965e-17, 1.759325738772092e-18, \
     -7.899853966841582e-17, -6.656660436056593e-17, -3.046782079812471e-17, \
@@ -492,8 +493,7 @@ BV_HD double bv_expneg_tab(double e, const double *tab) {
     1.8227458427912087e-17, -9.969531538920349e-17, 3.283107224245627e-17, \
     9.761887490727594e-17, -6.122763413004143e-17, 3.4034035352165297e-17, \
     -1.0619946056195963e-16, 1.0332385960676326e-16, 8.960767791036668e-17, \
-    4.0388753109278167e-17 \
-}
+    4.0388753109278167e-17
 BV_HD double bv_exp_tab_entry(int i) {
     const double hi[BV_EXP_TAB] = BV_EXP_TAB_HI;
     const double lo[BV_EXP_TAB] = BV_EXP_TAB_LO;

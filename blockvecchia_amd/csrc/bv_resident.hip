@@ -69,22 +69,6 @@ __device__ __forceinline__ void bv_lds_put(double *t, int lane, const bv_d4 &v) 
     *(bv_d4 *)(t + 4 * lane) = v;
 }
 
-/* sqrt(s), s >= 0: the hardware reciprocal square root and the Newton
-   corrections of the library square root, without its rescaling of inputs
-   below 2^-767 (squared distances of distinct points are far above) */
-__device__ __forceinline__ double bv_res_sqrt(double s) {
-    const double y = __builtin_amdgcn_rsq(s);
-    double g = s * y, h = 0.5 * y;
-    const double e = fma(-h, g, 0.5);
-    g = fma(g, e, g);
-    h = fma(h, e, h);
-    double d = fma(-g, g, s);
-    g = fma(d, h, g);
-    d = fma(-g, g, s);
-    g = fma(d, h, g);
-    return (s == 0.0) ? 0.0 : g;
-}
-
 /* Euclidean distance in the plane or in 3-D (dz = 0 in the plane adds an
    exact zero), products and sums rounded separately as the reference's
    sqrt(pow(dx,2) + pow(dy,2) [+ pow(dz,2)]). */
@@ -95,7 +79,7 @@ __device__ __forceinline__ double bv_res_dist(double xp, double yp, double zp, d
     const double dz = use_z ? zq - zp : 0.0;
     double s = dx * dx + dy * dy;
     s = s + dz * dz;
-    return bv_res_sqrt(s);
+    return bv_sqrt_nr(s);
 }
 
 /* A(r, c) in accumulator layout (lane: row 16 r + lane % 16, columns
