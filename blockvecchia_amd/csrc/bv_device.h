@@ -108,9 +108,20 @@ __device__ __forceinline__ double bv_sqrt_nr(double s) {
 static __device__ const double bv_exp_tab_dev[2 * BV_EXP_TAB] = {
     BV_EXP_TAB_HI_LIST, BV_EXP_TAB_LO_LIST};
 
-/* Closed-form Matern entry (nu = 1/2, 3/2, 5/2) between two points of the
-   plane: the distance as bv_dist_euclid rounds it, the square root and the
-   exponential above (within one ulp each), the forms of bv_matern_closed. */
+/* Whether a kind's entries take bv_matern_planar: Euclidean distances in the
+   plane, closed forms or general nu (the other kinds, 3-D and great-circle
+   distances take bv_cov). */
+template <int KIND>
+__device__ __forceinline__ bool bv_planar_ok(const bv_kparams &kp) {
+    return (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 || KIND == BV_KIND_NU25 ||
+            KIND == BV_KIND_GENERAL) && kp.metric == 0 && !kp.use_z;
+}
+
+/* Matern entry between two points of the plane: the distance as
+   bv_dist_euclid rounds it, the square root and the exponential above
+   (within one ulp each); the forms of bv_matern_closed (nu = 1/2, 3/2, 5/2)
+   or, for general nu, bv_matern_general_x (the evaluation's Chebyshev
+   table of x^nu e^x K_nu(x)). */
 template <int KIND>
 __device__ __forceinline__ double bv_matern_planar(double xa, double ya, double xb, double yb,
                                                    const bv_kparams &kp) {
@@ -118,6 +129,7 @@ __device__ __forceinline__ double bv_matern_planar(double xa, double ya, double 
     const double dx = xb - xa, dy = yb - ya;
     const double d = bv_sqrt_nr(dx * dx + dy * dy);
     const double t = d * kp.inv_scale;
+    if (KIND == BV_KIND_GENERAL) return bv_matern_general_x(t, kp, bv_exp_tab_dev);
     const double x = bv_expneg_tab(t, bv_exp_tab_dev);
     if (KIND == BV_KIND_NU05) return kp.sigma2 * x;
     if (KIND == BV_KIND_NU15) return kp.sigma2 * (1.0 + t) * x;

@@ -141,10 +141,9 @@ __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T
     const int J0 = j * BV_NB;
     const int rt0 = 2 * j;
     const int lr = lane & 15, lq = lane >> 4;
-    /* planar Euclidean closed forms: tiles away from the observation row and
-       the padding take a short path (wave-uniform test) */
-    const bool planar = (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 || KIND == BV_KIND_NU25) &&
-                        kp.metric == 0 && !kp.use_z;
+    /* planar Euclidean Matern entries: tiles away from the observation row
+       and the padding take a short path (wave-uniform test) */
+    const bool planar = bv_planar_ok<KIND>(kp);
     for (int rt = rtb; rt < T; rt += rstep) {
         const int p = rt * BV_TILE + lr;
         const double xp = sx[p], yp = sy[p], zp = sz[p];
@@ -887,8 +886,7 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems,
     const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
     {
         const bv_kparams &kp = s_kp;
-        const bool planar = (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 ||
-                             KIND == BV_KIND_NU25) && kp.metric == 0 && !kp.use_z;
+        const bool planar = bv_planar_ok<KIND>(kp);
 #pragma unroll
         for (int s = 0; s < G; ++s) {
             acc[s][0] = zero;
@@ -1079,8 +1077,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
         bv_wave_sync();
     }
     const bv_kparams &kp = s_kp;
-    const bool planar = (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 || KIND == BV_KIND_NU25) &&
-                        kp.metric == 0 && !kp.use_z;
+    const bool planar = bv_planar_ok<KIND>(kp);
     const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
 
     bv_d4 xl[G][2]; /* L(R, panel j) */
@@ -1465,8 +1462,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
     }
     __syncthreads();
     const bv_kparams &kp = s_kp;
-    const bool planar = (KIND == BV_KIND_NU05 || KIND == BV_KIND_NU15 || KIND == BV_KIND_NU25) &&
-                        kp.metric == 0 && !kp.use_z;
+    const bool planar = bv_planar_ok<KIND>(kp);
     const bv_d4 zero = (bv_d4){0.0, 0.0, 0.0, 0.0};
 
     bv_d4 d00 = zero, d10 = zero, d11 = zero; /* own rows' SYRK over k < c0 */

@@ -353,9 +353,17 @@ BV_HD double bv_cheb_coef(const double *gv, int t) {
 }
 
 /* x^nu K_nu(x), x > 0, from the table. */
+BV_HD double bv_cheb_g(double x, const double *cheb);
+
 BV_HD double bv_xnu_knu_cheb(double x, const double *cheb, double nu) {
     if (!(x >= 0x1p-24)) return bv_xnu_knu_slow(x, nu);
     if (x >= 0x1p10) return 0.0;
+    return bv_cheb_g(x, cheb) * exp(-x);
+}
+
+/* g(x) = x^nu e^x K_nu(x) for x in [2^-24, 2^10) from the table: the
+   half-octave from the exponent bits, Clenshaw's recurrence on it. */
+BV_HD double bv_cheb_g(double x, const double *cheb) {
     uint64_t b;
     __builtin_memcpy(&b, &x, sizeof(b));
     const int e = (int)((b >> 52) & 0x7FF) - 1023;
@@ -374,8 +382,7 @@ BV_HD double bv_xnu_knu_cheb(double x, const double *cheb, double nu) {
         b2 = b1;
         b1 = b0;
     }
-    const double g = t * b1 - b2 + c[0];
-    return g * exp(-x);
+    return t * b1 - b2 + c[0];
 }
 
 /* Great-circle distance (src/ckernel.cpp:145-154). */
@@ -441,6 +448,22 @@ BV_HD double bv_expneg_tab(double e, const double *tab) {
     const int n = -(int)kd;
     const double th = tab[n & 63], tl = tab[BV_EXP_TAB + (n & 63)];
     return ldexp(th + fma(th, pm1, tl), n >> 6);
+}
+
+/* x^nu K_nu(x) as bv_xnu_knu_cheb, with e^-x by the table-driven
+   exponential above (tab: its 2 x 64 entries) instead of exp. */
+BV_HD double bv_xnu_knu_cheb_t(double x, const double *cheb, double nu, const double *tab) {
+    if (!(x >= 0x1p-24)) return bv_xnu_knu_slow(x, nu);
+    if (x >= 0x1p10) return 0.0;
+    return bv_cheb_g(x, cheb) * bv_expneg_tab(x, tab);
+}
+
+/* General-nu entry at the scaled distance x = d * inv_scale (the reciprocal
+   of beta * dist_scale, as the closed forms scale; within 2 ulp of the
+   reference's two divisions): sigma^2 at 0, else con x^nu K_nu(x). */
+BV_HD double bv_matern_general_x(double x, const bv_kparams &kp, const double *tab) {
+    if (x == 0.0) return kp.sigma2;
+    return kp.con * bv_xnu_knu_cheb_t(x, kp.cheb, kp.nu, tab);
 }
 
 /* T_j = 2^(j/64) rounded to nearest and L_j = 2^(j/64) - T_j rounded, from

@@ -107,10 +107,8 @@ __device__ __forceinline__ bv_d4 bv_res_gen(const bv_res_block &B, int r, int c,
             const double d = bv_res_dist(xp, yp, zp, B.sx[q], B.sy[q], B.sz[q], use_z);
             double e;
             if (KIND == BV_KIND_GENERAL) {
-                /* x = d / beta / dist_scale as the reference divides */
-                double x = d / kp.beta;
-                x = x / kp.dist_scale;
-                e = (x == 0.0) ? kp.sigma2 : kp.con * bv_xnu_knu_cheb(x, kp.cheb, kp.nu);
+                /* the entries of bv_matern_planar, with the table in LDS */
+                e = bv_matern_general_x(d * kp.inv_scale, kp, B.etab);
             } else {
                 /* the closed forms of bv_matern_closed with the table exponential */
                 const double t = d * kp.inv_scale;

@@ -33,6 +33,15 @@ extern "C" void shim_cheb_table(double nu, double *tab) {
 extern "C" double shim_xnu_knu_cheb(double nu, double x, const double *tab) {
     return bv_xnu_knu_cheb(x, tab, nu);
 }
+extern "C" double shim_xnu_knu_cheb_t(double nu, double x, const double *tab) {
+    static double et[2 * BV_EXP_TAB];
+    static int init = 0;
+    if (!init) {
+        for (int i = 0; i < 2 * BV_EXP_TAB; ++i) et[i] = bv_exp_tab_entry(i);
+        init = 1;
+    }
+    return bv_xnu_knu_cheb_t(x, tab, nu, et);
+}
 extern "C" double shim_exp_tab_entry(int i) { return bv_exp_tab_entry(i); }
 extern "C" void shim_expneg_tab(const double *e, double *out, long n) {
     double tab[2 * BV_EXP_TAB];
@@ -86,6 +95,8 @@ def shim(tmp_path_factory):
     lib.shim_cheb_table.restype = None
     lib.shim_xnu_knu_cheb.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
     lib.shim_xnu_knu_cheb.restype = ctypes.c_double
+    lib.shim_xnu_knu_cheb_t.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+    lib.shim_xnu_knu_cheb_t.restype = ctypes.c_double
     lib.shim_exp_tab_entry.argtypes = [ctypes.c_int]
     lib.shim_exp_tab_entry.restype = ctypes.c_double
     lib.shim_expneg_tab.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
@@ -167,13 +178,20 @@ def _cheb(shim, nu, cache={}):
     return cache[nu]
 
 
-def test_chebyshev_knu_golden_and_random(shim):
+CHEB_FNS = ["shim_xnu_knu_cheb", "shim_xnu_knu_cheb_t"]
+
+
+@pytest.mark.parametrize("fn", CHEB_FNS)
+def test_chebyshev_knu_golden_and_random(shim, fn):
     """x^nu K_nu(x) from the per-nu Chebyshev table (the path every general-nu
     entry takes: 14 terms on each half-octave of [2^-24, 2^10), coefficients
     from the accurate routine at the Chebyshev nodes) against the 40-digit
     mpmath table and, on random (nu, x), the oracle's quadrature K_nu.  The
     table's coefficients here are built on the host by the same functions the
-    device's bv_cheb_build_kernel calls."""
+    device's bv_cheb_build_kernel calls.  Both forms: e^-x by exp
+    (bv_xnu_knu_cheb) and by the table-driven exponential (bv_xnu_knu_cheb_t,
+    the factorization kernels' planar entries)."""
+    knu = getattr(shim, fn)
     import math
     with open(os.path.join(HERE, "golden", "besselk_golden.json")) as f:
         rows = json.load(f)["rows"]
@@ -185,7 +203,7 @@ def test_chebyshev_knu_golden_and_random(shim):
         r = math.exp(nu * math.log(x)) * ref
         if r == 0.0 or not math.isfinite(r) or r < 1e-300:
             continue
-        v = shim.shim_xnu_knu_cheb(nu, x, _cheb(shim, nu).ctypes.data)
+        v = knu(nu, x, _cheb(shim, nu).ctypes.data)
         worst = max(worst, abs(v - r) / r)
         n += 1
     assert n > 300
@@ -195,12 +213,13 @@ def test_chebyshev_knu_golden_and_random(shim):
         nu = float(rng.uniform(0.01, 3.0))
         x = float(10 ** rng.uniform(-9, np.log10(650)))
         tab = _cheb(shim, nu)
-        a = shim.shim_xnu_knu_cheb(nu, x, tab.ctypes.data)
+        a = knu(nu, x, tab.ctypes.data)
         b = math.exp(nu * math.log(x)) * oracle.besselk(nu, x)
         assert abs(a - b) <= 2e-14 * b, (nu, x, a, b)
 
 
-def test_chebyshev_knu_half_integer_closed_forms(shim):
+@pytest.mark.parametrize("fn", CHEB_FNS)
+def test_chebyshev_knu_half_integer_closed_forms(shim, fn):
     """For nu = 1/2, 3/2, 5/2 the table must reproduce the elementary forms
     x^nu K_nu(x) = sqrt(pi/2) e^-x p(x), p = 1, 1 + x, 3 + 3x + x^2 (a
     dropped or misplaced node or coefficient fails this on whole
@@ -214,7 +233,7 @@ def test_chebyshev_knu_half_integer_closed_forms(shim):
             ref = math.sqrt(math.pi / 2) * math.exp(-x) * p(x)
             if ref < 1e-300:
                 continue
-            v = shim.shim_xnu_knu_cheb(nu, x, tab.ctypes.data)
+            v = getattr(shim, fn)(nu, x, tab.ctypes.data)
             assert abs(v - ref) <= 1e-14 * ref, (nu, x, v, ref)
 
 
