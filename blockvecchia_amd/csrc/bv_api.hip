@@ -545,17 +545,38 @@ extern "C" int bv_create(const bv_problem *p, bv_ctx **out) {
     return BV_OK;
 }
 
+/* The register-resident builds of nblk descriptors sorted by padded size,
+   largest first: runs of equal bv_res_nr (at most BV_RES_NR_MAX - 1), build
+   nr[i] for cnt[i] descriptors; returns the number of runs. */
+static int bv_res_runs(const bv_block_desc *h_desc, unsigned nblk, int *nr, unsigned *cnt) {
+    int n = 0;
+    for (unsigned i = 0; i < nblk; ++i) {
+        const int r = bv_res_nr(h_desc[i].mp);
+        if (n == 0 || nr[n - 1] != r) {
+            nr[n] = r;
+            cnt[n] = 0;
+            ++n;
+        }
+        ++cnt[n - 1];
+    }
+    return n;
+}
+
 /* The register-resident kernel over nblk descriptors sorted by padded size,
-   largest first: those above 16 * 20 with the 24-row build, the rest with the
-   20-row build (fewer registers live), on one stream. */
+   largest first: each run with its build (the fewest registers that hold its
+   blocks), on one stream. */
 static hipError_t bv_launch_res_sorted(int kind, const bv_block_desc *h_desc,
                                        const bv_block_desc *d_desc, unsigned nblk,
                                        hipStream_t st, const bv_eval_slot *slots, double *ws) {
-    unsigned n6 = 0;
-    while (n6 < nblk && h_desc[n6].mp > BV_RES_MP_SMALL) ++n6;
-    hipError_t e = bv_launch_res(kind, 6, n6, st, d_desc, slots, ws);
-    if (e != hipSuccess) return e;
-    return bv_launch_res(kind, 5, nblk - n6, st, d_desc + n6, slots, ws);
+    int nr[BV_RES_NR_MAX];
+    unsigned cnt[BV_RES_NR_MAX], o = 0;
+    const int n = bv_res_runs(h_desc, nblk, nr, cnt);
+    for (int i = 0; i < n; ++i) {
+        const hipError_t e = bv_launch_res(kind, nr[i], cnt[i], st, d_desc + o, slots, ws);
+        if (e != hipSuccess) return e;
+        o += cnt[i];
+    }
+    return hipSuccess;
 }
 
 /* The kernels of one call after its slots are on the device: covariance
@@ -624,22 +645,24 @@ static int bv_enqueue(bv_ctx *c, unsigned nblk, int kinds_used) {
             /* descriptors of k evaluations: each block's k copies in a row,
                blocks sorted by padded size */
             const unsigned k = (unsigned)c->desc_evals;
-            unsigned n6 = 0;
-            while (n6 < (unsigned)c->B && c->base_desc[n6].mp > BV_RES_MP_SMALL) ++n6;
-            n6 *= k;
-            if (n6 > 0 && n6 < nblk && c->cstream[0]) {
-                /* the few large blocks on a side stream beside the others */
+            int nr[BV_RES_NR_MAX];
+            unsigned cnt[BV_RES_NR_MAX], o = 0;
+            const int n = bv_res_runs(c->base_desc.data(), (unsigned)c->B, nr, cnt);
+            const bool side = n > 1 && c->cstream[0];
+            if (side) {
+                /* the run of the largest blocks on a side stream beside the others */
                 BV_HIP(hipEventRecord(c->cev[1], c->stream));
                 BV_HIP(hipStreamWaitEvent(c->cstream[0], c->cev[1], 0));
-                BV_HIP(bv_launch_res(kind, 6, n6, c->cstream[0], c->d_desc, c->d_slots, c->d_ws));
-                BV_HIP(bv_launch_res(kind, 5, nblk - n6, c->stream, c->d_desc + n6, c->d_slots,
-                                     c->d_ws));
+            }
+            for (int i = 0; i < n; ++i) {
+                const unsigned m = cnt[i] * k;
+                BV_HIP(bv_launch_res(kind, nr[i], m, (side && i == 0) ? c->cstream[0] : c->stream,
+                                     c->d_desc + o, c->d_slots, c->d_ws));
+                o += m;
+            }
+            if (side) {
                 BV_HIP(hipEventRecord(c->cev[0], c->cstream[0]));
                 BV_HIP(hipStreamWaitEvent(c->stream, c->cev[0], 0));
-            } else {
-                BV_HIP(bv_launch_res(kind, 6, n6, c->stream, c->d_desc, c->d_slots, c->d_ws));
-                BV_HIP(bv_launch_res(kind, 5, nblk - n6, c->stream, c->d_desc + n6, c->d_slots,
-                                     c->d_ws));
             }
         } else if (c->variant == 1)
             BV_HIP(bv_launch_block_1w(kind, nblk, lds, c->stream, c->d_desc, c->d_slots, c->d_ws));

@@ -32,7 +32,11 @@
 #endif
 #define BV_MAX_MP 1024    /* largest padded augmented size handled (LDS staging)  */
 #define BV_RES_MAX_MP 384 /* register-resident kernel: padded sizes up to 16 * 24 */
-#define BV_RES_MP_SMALL 320 /* ... its 20-row build takes padded sizes up to this */
+#define BV_RES_NR_MAX 6   /* ... in builds of 4 NR tile rows, NR = 2 .. 6           */
+
+/* The register-resident build for a block of padded size mp: the smallest
+   NR >= 2 with 64 NR >= mp. */
+static inline int bv_res_nr(int mp) { return mp <= 128 ? 2 : (mp + 63) / 64; }
 
 /* One block of one evaluation, in launch order. */
 struct bv_block_desc {

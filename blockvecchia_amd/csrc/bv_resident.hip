@@ -27,11 +27,14 @@
  *
  * LDS: two slots of 4 NR tiles (slot j % 2: pivot row j's tiles at t < j,
  * column j's covariance tiles at t >= j), two W slots, the staged points
- * (4 x 64 NR doubles), the exponential's table: 97 KB (NR = 5) or 113 KB
- * (NR = 6).  One workgroup per compute unit, four wavefronts of up to 512
- * registers each, one per SIMD.  Rows a wavefront holds only statically
- * (finished, or beyond the block) are skipped by wave-uniform branches or
- * carry values nobody reads.
+ * (4 x 64 NR doubles), the exponential's table: 42, 61, 79, 97 and 113 KB for
+ * NR = 2 .. 6.  NR = 2 and 3 (padded sizes up to 128 and 192) run two
+ * workgroups per compute unit (two wavefronts per SIMD, up to 256 registers
+ * each: one block's vector work, covariance generation and diagonal sweeps,
+ * overlaps the other's matrix-core products); NR = 4 .. 6 one (a wavefront's
+ * live tiles need up to 512 registers).  Rows a wavefront holds only
+ * statically (finished, or beyond the block) are skipped by wave-uniform
+ * branches or carry values nobody reads.
  */
 #include <hip/hip_runtime.h>
 #include "bv_engine.h"
@@ -379,7 +382,7 @@ __device__ __forceinline__ void bv_res_col(bv_res_state<NR> &S, const bv_res_blo
 } // namespace
 
 template <int KIND, int NR>
-__global__ void __launch_bounds__(BV_RES_THREADS, 1)
+__global__ void __launch_bounds__(BV_RES_THREADS, (NR <= 3) ? 2 : 1)
 bv_res_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__restrict__ slots,
               const double *ws_base) {
     constexpr int NC = 4 * NR;
@@ -484,17 +487,32 @@ template <int KIND>
 hipError_t bv_launch_res_k(int nr, unsigned nblk, hipStream_t st, const bv_block_desc *desc,
                            const bv_eval_slot *slots, double *ws);
 
-/* Launch over nblk descriptors whose blocks all have at most 4 NR tile rows:
-   NR = 5 (padded size <= 320) or 6 (<= 384). */
+/* Launch over nblk descriptors whose blocks all have at most 4 NR tile rows
+   (padded size <= 64 NR), NR = 2 .. 6. */
 template <>
 hipError_t bv_launch_res_k<BV_RES_KIND>(int nr, unsigned nblk, hipStream_t st,
                                         const bv_block_desc *desc, const bv_eval_slot *slots,
                                         double *ws) {
     if (nblk == 0) return hipSuccess;
     const dim3 grid(nblk), blk(BV_RES_THREADS);
-    if (nr == 5)
+    switch (nr) {
+    case 2:
+        hipLaunchKernelGGL((bv_res_kernel<BV_RES_KIND, 2>), grid, blk, 0, st, desc, slots, ws);
+        break;
+    case 3:
+        hipLaunchKernelGGL((bv_res_kernel<BV_RES_KIND, 3>), grid, blk, 0, st, desc, slots, ws);
+        break;
+    case 4:
+        hipLaunchKernelGGL((bv_res_kernel<BV_RES_KIND, 4>), grid, blk, 0, st, desc, slots, ws);
+        break;
+    case 5:
         hipLaunchKernelGGL((bv_res_kernel<BV_RES_KIND, 5>), grid, blk, 0, st, desc, slots, ws);
-    else
+        break;
+    case 6:
         hipLaunchKernelGGL((bv_res_kernel<BV_RES_KIND, 6>), grid, blk, 0, st, desc, slots, ws);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }

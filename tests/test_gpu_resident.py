@@ -1,9 +1,10 @@
 """The register-resident factorization (bv_res_kernel, BV_FACTOR_KERNEL=res):
 one workgroup per block with the factor held in registers.  Against the
-oracle on the shapes its two builds cover (padded sizes up to 320 and up to
-384, both in one evaluation on two streams), every covariance kind it takes
-(closed forms, general nu, 3-D distances, space-time read from the
-workspace), the non-positive-definite report, and its limits."""
+oracle on the shapes its builds cover (padded sizes up to 128, 192, 256, 320
+and 384; several builds in one evaluation, the largest blocks' on a second
+stream), every covariance kind it takes (closed forms, general nu, 3-D
+distances, space-time read from the workspace), the non-positive-definite
+report, and its limits."""
 import math
 import sys
 
@@ -69,6 +70,32 @@ def test_resident_shapes(cs, theta, block_rtol):
     b = bv.partition(x, y, obs, 100, cs, seed=0)
     _say("case: n=1500 bc=100 cs=%d theta=%s padded <= %d" % (cs, theta, _max_mp(b, cs)))
     _check(b, theta, block_rtol=block_rtol)
+
+
+def _builds(blocks, cs):
+    m = blocks.batch_num.astype(np.int64) + cs
+    m[0] = blocks.batch_num[0]
+    mp = (m + 1 + 31) // 32 * 32
+    nr = np.where(mp <= 128, 2, (mp + 63) // 64)
+    u, c = np.unique(nr, return_counts=True)
+    return dict(zip(u.tolist(), c.tolist()))
+
+
+@pytest.mark.parametrize("n,bc,cs,theta", [
+    (1500, 100, 30, [1.0, 0.1, 0.5]),    # 8-row build (two workgroups per compute unit)
+    (1500, 60, 60, [1.2, 0.08, 1.5]),    # 8-row build, larger blocks
+    (2500, 125, 150, [1.0, 0.08, 0.8]),  # 12-row build (and 16 for a few), general nu
+    (2500, 100, 200, [1.0, 0.1, 0.5]),   # 16-row build (and 12 / 20 around it)
+])
+def test_resident_builds(n, bc, cs, theta):
+    """Each build of the resident kernel, from 8 to 16 tile rows, against the
+    oracle, with the blocks of one evaluation split over several builds."""
+    x, y = datagen.uniform_locations(n, 8)
+    obs = datagen.sample_dense(x, y, *theta, seed=8)
+    b = bv.partition(x, y, obs, bc, cs, seed=0)
+    _say("case: n=%d bc=%d cs=%d theta=%s blocks per build (4 NR rows) %s"
+         % (n, bc, cs, theta, _builds(b, cs)))
+    _check(b, theta)
 
 
 def test_resident_3d_and_dist_scale():
