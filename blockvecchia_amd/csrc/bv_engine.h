@@ -33,6 +33,7 @@
 #define BV_MAX_MP 1024    /* largest padded augmented size handled (LDS staging)  */
 #define BV_RES_MAX_MP 384 /* register-resident kernel: padded sizes up to 16 * 24 */
 #define BV_RES_NR_MAX 6   /* ... in builds of 4 NR tile rows, NR = 2 .. 6           */
+#define BV_RES_TUNE_MIN_MP 129 /* ... offered by the autotuner above this size */
 
 /* The register-resident build for a block of padded size mp: the smallest
    NR >= 2 with 64 NR >= mp. */
