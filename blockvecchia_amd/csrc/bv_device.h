@@ -120,16 +120,16 @@ __device__ __forceinline__ bool bv_planar_ok(const bv_kparams &kp) {
 /* Matern entry between two points of the plane: the distance as
    bv_dist_euclid rounds it, the square root and the exponential above
    (within one ulp each); the forms of bv_matern_closed (nu = 1/2, 3/2, 5/2)
-   or, for general nu, bv_matern_general_x (the evaluation's Chebyshev
-   table of x^nu e^x K_nu(x)). */
+   or, for general nu, bv_matern_general_x (cheb: the evaluation's Chebyshev
+   table of x^nu e^x K_nu(x), kp.cheb or a copy in LDS). */
 template <int KIND>
 __device__ __forceinline__ double bv_matern_planar(double xa, double ya, double xb, double yb,
-                                                   const bv_kparams &kp) {
+                                                   const bv_kparams &kp, const double *cheb) {
 #pragma clang fp contract(off)
     const double dx = xb - xa, dy = yb - ya;
     const double d = bv_sqrt_nr(dx * dx + dy * dy);
     const double t = d * kp.inv_scale;
-    if (KIND == BV_KIND_GENERAL) return bv_matern_general_x(t, kp, bv_exp_tab_dev);
+    if (KIND == BV_KIND_GENERAL) return bv_matern_general_x(t, kp, cheb, bv_exp_tab_dev);
     const double x = bv_expneg_tab(t, bv_exp_tab_dev);
     if (KIND == BV_KIND_NU05) return kp.sigma2 * x;
     if (KIND == BV_KIND_NU15) return kp.sigma2 * (1.0 + t) * x;

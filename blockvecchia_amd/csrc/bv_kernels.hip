@@ -155,7 +155,7 @@ __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
                     const int q = q0 + 4 * g + lq;
-                    v[g] = bv_matern_planar<KIND>(xp, yp, sx[q], sy[q], kp);
+                    v[g] = bv_matern_planar<KIND>(xp, yp, sx[q], sy[q], kp, kp.cheb);
                 }
             } else {
 #pragma unroll
@@ -907,7 +907,8 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems,
 #pragma unroll
                         for (int g = 0; g < 4; ++g) {
                             const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
-                            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp);
+                            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp,
+                                                          kp.cheb);
                         }
                     } else {
 #pragma unroll
@@ -1108,7 +1109,8 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
 #pragma unroll
                         for (int g = 0; g < 4; ++g) {
                             const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
-                            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp);
+                            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp,
+                                                          kp.cheb);
                         }
                     } else {
 #pragma unroll
@@ -1192,7 +1194,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
                         const int iq = ct * BV_TILE + 4 * g + lq; /* own rows are the columns */
-                        v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp);
+                        v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp, kp.cheb);
                     }
                 } else {
 #pragma unroll
@@ -1350,7 +1352,8 @@ __device__ __forceinline__ bv_d4 bv_pair_tile(const double *__restrict__ ws, int
                                               int m, int ip0, int iq0, int lane,
                                               const double *qx, const double *qy,
                                               const double *qz, const double *qo,
-                                              const bv_kparams &kp, bool planar) {
+                                              const bv_kparams &kp, bool planar,
+                                              const double *cheb) {
     if (KIND == BV_KIND_LOADA) return *(const bv_d4 *)(ws + bv_tile_off(rt, ct, T) + 4 * lane);
     const int lr = lane & 15, lq = lane >> 4;
     const int p = rt * BV_TILE + lr, ip = ip0 + lr, q0 = ct * BV_TILE;
@@ -1359,7 +1362,7 @@ __device__ __forceinline__ bv_d4 bv_pair_tile(const double *__restrict__ ws, int
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int iq = iq0 + 4 * g + lq;
-            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp);
+            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp, cheb);
         }
     } else {
 #pragma unroll
@@ -1430,6 +1433,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
     __shared__ double s_logdet[2], s_dotd[2];
     __shared__ int s_info[2];
     __shared__ bv_kparams s_kp;
+    __shared__ double s_cheb[KIND == BV_KIND_GENERAL ? BV_CHEB_SIZE : 1]; /* kp.cheb, copied */
 
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
@@ -1459,6 +1463,8 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
             qx[tid] = xv; qy[tid] = yv; qz[tid] = zv; qo[tid] = ov;
         }
         if (tid == 0) s_kp = E.kp;
+        if (KIND == BV_KIND_GENERAL)
+            for (int i = tid; i < BV_CHEB_SIZE; i += 128) s_cheb[i] = E.kp.cheb[i];
     }
     __syncthreads();
     const bv_kparams &kp = s_kp;
@@ -1472,7 +1478,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
            dynamically (scratch memory) */
         auto gen = [&](int i, int c) {
             return -bv_pair_tile<KIND>(ws, T, myr + i, c0 + c, m, (2 * w + i) * BV_TILE,
-                                       64 + c * BV_TILE, lane, qx, qy, qz, qo, kp, planar);
+                                       64 + c * BV_TILE, lane, qx, qy, qz, qo, kp, planar, s_cheb);
         };
         bv_d4 acc[2][4] = {{gen(0, 0), gen(0, 1), gen(0, 2), gen(0, 3)},
                            {gen(1, 0), gen(1, 1), gen(1, 2), gen(1, 3)}};
@@ -1551,11 +1557,12 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
     bv_d4 db0 = zero, db1 = zero, db2 = zero; /* wave B: D_BB */
     bv_d4 e[2][2];                            /* wave B: D_BA */
     if (w == 0) {
-        bv_d4 da0 = d00 - bv_pair_tile<KIND>(ws, T, qn, qn, m, 0, 0, lane, qx, qy, qz, qo, kp, planar);
+        bv_d4 da0 = d00 - bv_pair_tile<KIND>(ws, T, qn, qn, m, 0, 0, lane, qx, qy, qz, qo, kp,
+                                             planar, s_cheb);
         bv_d4 da1 = d10 - bv_pair_tile<KIND>(ws, T, qn + 1, qn, m, 16, 0, lane, qx, qy, qz, qo, kp,
-                                             planar);
+                                             planar, s_cheb);
         bv_d4 da2 = d11 - bv_pair_tile<KIND>(ws, T, qn + 1, qn + 1, m, 16, 16, lane, qx, qy, qz, qo,
-                                             kp, planar);
+                                             kp, planar, s_cheb);
         if (s >= 0) bv_pair_syrk_own(ws, T, myr, c0, lane, da0, da1, da2);
         *(bv_d4 *)(sW[0] + 0 * 256 + 4 * lane) = -da0;
         *(bv_d4 *)(sW[0] + 1 * 256 + 4 * lane) = -da1;
@@ -1563,11 +1570,11 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
         if (lane == 0) { s_logdet[0] = 0.0; s_dotd[0] = 0.0; s_info[0] = 0; }
     } else if (hasB) {
         db0 = d00 - bv_pair_tile<KIND>(ws, T, qn + 2, qn + 2, m, 32, 32, lane, qx, qy, qz, qo, kp,
-                                       planar);
+                                       planar, s_cheb);
         db1 = d10 - bv_pair_tile<KIND>(ws, T, qn + 3, qn + 2, m, 48, 32, lane, qx, qy, qz, qo, kp,
-                                       planar);
+                                       planar, s_cheb);
         db2 = d11 - bv_pair_tile<KIND>(ws, T, qn + 3, qn + 3, m, 48, 48, lane, qx, qy, qz, qo, kp,
-                                       planar);
+                                       planar, s_cheb);
         if (s >= 0) bv_pair_syrk_own(ws, T, myr, c0, lane, db0, db1, db2);
         if (lane == 0) { s_logdet[1] = 0.0; s_dotd[1] = 0.0; s_info[1] = 0; }
     }
@@ -1588,7 +1595,8 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
 #pragma unroll
             for (int c = 0; c < 2; ++c)
                 e[i][c] = -bv_pair_tile<KIND>(ws, T, qn + 2 + i, qn + c, m, (2 + i) * BV_TILE,
-                                              c * BV_TILE, lane, qx, qy, qz, qo, kp, planar);
+                                              c * BV_TILE, lane, qx, qy, qz, qo, kp, planar,
+                                              s_cheb);
         }
         if (s >= 0) {
             bv_step_gemm<2, 2>(ws, T, qn / 2, c0, qn + 2, lane, e);

@@ -460,10 +460,12 @@ BV_HD double bv_xnu_knu_cheb_t(double x, const double *cheb, double nu, const do
 
 /* General-nu entry at the scaled distance x = d * inv_scale (the reciprocal
    of beta * dist_scale, as the closed forms scale; within 2 ulp of the
-   reference's two divisions): sigma^2 at 0, else con x^nu K_nu(x). */
-BV_HD double bv_matern_general_x(double x, const bv_kparams &kp, const double *tab) {
+   reference's two divisions): sigma^2 at 0, else con x^nu K_nu(x), from the
+   evaluation's table cheb (kp.cheb or a copy of it). */
+BV_HD double bv_matern_general_x(double x, const bv_kparams &kp, const double *cheb,
+                                 const double *tab) {
     if (x == 0.0) return kp.sigma2;
-    return kp.con * bv_xnu_knu_cheb_t(x, kp.cheb, kp.nu, tab);
+    return kp.con * bv_xnu_knu_cheb_t(x, cheb, kp.nu, tab);
 }
 
 /* T_j = 2^(j/64) rounded to nearest and L_j = 2^(j/64) - T_j rounded, from

@@ -59,6 +59,7 @@ struct bv_res_block {
     int T, NC, m, cs0, w, lane;
     const double *sx, *sy, *sz, *so;  /* staged points of the block (LDS)        */
     const double *etab;               /* 2^(j/64) as hi, lo pairs (LDS)          */
+    const double *cheb;               /* general nu: the x^nu K_nu table (LDS)   */
     double *buf;                      /* two slots of pivot-row / column tiles   */
     double *sW;                       /* two W slots (LDS)                       */
     const double *ws;                 /* LOADA: the block's A tiles (workspace)  */
@@ -110,8 +111,8 @@ __device__ __forceinline__ bv_d4 bv_res_gen(const bv_res_block &B, int r, int c,
             const double d = bv_res_dist(xp, yp, zp, B.sx[q], B.sy[q], B.sz[q], use_z);
             double e;
             if (KIND == BV_KIND_GENERAL) {
-                /* the entries of bv_matern_planar, with the table in LDS */
-                e = bv_matern_general_x(d * kp.inv_scale, kp, B.etab);
+                /* the entries of bv_matern_planar, with the tables in LDS */
+                e = bv_matern_general_x(d * kp.inv_scale, kp, B.cheb, B.etab);
             } else {
                 /* the closed forms of bv_matern_closed with the table exponential */
                 const double t = d * kp.inv_scale;
@@ -392,6 +393,7 @@ bv_res_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__rest
     __shared__ double s_x[NP], s_y[NP], s_z[NP], s_o[NP];
     __shared__ double s_red[4][2];
     __shared__ double s_etab[2 * BV_EXP_TAB];
+    __shared__ double s_cheb[KIND == BV_KIND_GENERAL ? BV_CHEB_SIZE : 1];
     __shared__ int s_redi[4];
 
     const bv_block_desc d = desc[blockIdx.x];
@@ -409,6 +411,7 @@ bv_res_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__rest
     B.lane = tid & 63;
     B.sx = s_x; B.sy = s_y; B.sz = s_z; B.so = s_o;
     B.etab = s_etab;
+    B.cheb = s_cheb;
     B.buf = s_buf;
     B.sW = s_W;
     B.ws = ws_base + d.ws_off;
@@ -422,6 +425,8 @@ bv_res_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__rest
         return;
     }
     if (tid < 2 * BV_EXP_TAB) s_etab[tid] = bv_exp_tab_entry(tid);
+    if (KIND == BV_KIND_GENERAL)
+        for (int i = tid; i < BV_CHEB_SIZE; i += BV_RES_THREADS) s_cheb[i] = E.kp.cheb[i];
     if (tid == 0) s_kp = E.kp;
     if (KIND != BV_KIND_LOADA) {
         for (int p = tid; p < NP; p += BV_RES_THREADS) {
