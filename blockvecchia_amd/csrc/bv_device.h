@@ -121,16 +121,18 @@ __device__ __forceinline__ bool bv_planar_ok(const bv_kparams &kp) {
    bv_dist_euclid rounds it, the square root and the exponential above
    (within one ulp each); the forms of bv_matern_closed (nu = 1/2, 3/2, 5/2)
    or, for general nu, bv_matern_general_x (cheb: the evaluation's Chebyshev
-   table of x^nu e^x K_nu(x), kp.cheb or a copy in LDS). */
+   table of x^nu e^x K_nu(x), kp.cheb or a copy in LDS; etab: the
+   exponential's table, bv_exp_tab_dev or a copy in LDS). */
 template <int KIND>
 __device__ __forceinline__ double bv_matern_planar(double xa, double ya, double xb, double yb,
-                                                   const bv_kparams &kp, const double *cheb) {
+                                                   const bv_kparams &kp, const double *cheb,
+                                                   const double *etab) {
 #pragma clang fp contract(off)
     const double dx = xb - xa, dy = yb - ya;
     const double d = bv_sqrt_nr(dx * dx + dy * dy);
     const double t = d * kp.inv_scale;
-    if (KIND == BV_KIND_GENERAL) return bv_matern_general_x(t, kp, cheb, bv_exp_tab_dev);
-    const double x = bv_expneg_tab(t, bv_exp_tab_dev);
+    if (KIND == BV_KIND_GENERAL) return bv_matern_general_x(t, kp, cheb, etab);
+    const double x = bv_expneg_tab(t, etab);
     if (KIND == BV_KIND_NU05) return kp.sigma2 * x;
     if (KIND == BV_KIND_NU15) return kp.sigma2 * (1.0 + t) * x;
     return kp.sigma2 * (1.0 + t + t * t * (1.0 / 3.0)) * x;

@@ -155,7 +155,8 @@ __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
                     const int q = q0 + 4 * g + lq;
-                    v[g] = bv_matern_planar<KIND>(xp, yp, sx[q], sy[q], kp, kp.cheb);
+                    v[g] = bv_matern_planar<KIND>(xp, yp, sx[q], sy[q], kp, kp.cheb,
+                                                  bv_exp_tab_dev);
                 }
             } else {
 #pragma unroll
@@ -908,7 +909,7 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems,
                         for (int g = 0; g < 4; ++g) {
                             const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
                             v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp,
-                                                          kp.cheb);
+                                                          kp.cheb, bv_exp_tab_dev);
                         }
                     } else {
 #pragma unroll
@@ -1110,7 +1111,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
                         for (int g = 0; g < 4; ++g) {
                             const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
                             v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp,
-                                                          kp.cheb);
+                                                          kp.cheb, bv_exp_tab_dev);
                         }
                     } else {
 #pragma unroll
@@ -1194,7 +1195,8 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
                         const int iq = ct * BV_TILE + 4 * g + lq; /* own rows are the columns */
-                        v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp, kp.cheb);
+                        v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp,
+                                                      kp.cheb, bv_exp_tab_dev);
                     }
                 } else {
 #pragma unroll
@@ -1353,7 +1355,7 @@ __device__ __forceinline__ bv_d4 bv_pair_tile(const double *__restrict__ ws, int
                                               const double *qx, const double *qy,
                                               const double *qz, const double *qo,
                                               const bv_kparams &kp, bool planar,
-                                              const double *cheb) {
+                                              const double *cheb, const double *etab) {
     if (KIND == BV_KIND_LOADA) return *(const bv_d4 *)(ws + bv_tile_off(rt, ct, T) + 4 * lane);
     const int lr = lane & 15, lq = lane >> 4;
     const int p = rt * BV_TILE + lr, ip = ip0 + lr, q0 = ct * BV_TILE;
@@ -1362,7 +1364,7 @@ __device__ __forceinline__ bv_d4 bv_pair_tile(const double *__restrict__ ws, int
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int iq = iq0 + 4 * g + lq;
-            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp, cheb);
+            v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp, cheb, etab);
         }
     } else {
 #pragma unroll
@@ -1434,6 +1436,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
     __shared__ int s_info[2];
     __shared__ bv_kparams s_kp;
     __shared__ double s_cheb[KIND == BV_KIND_GENERAL ? BV_CHEB_SIZE : 1]; /* kp.cheb, copied */
+    __shared__ double s_etab[2 * BV_EXP_TAB];                             /* bv_exp_tab_dev   */
 
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
@@ -1465,6 +1468,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
         if (tid == 0) s_kp = E.kp;
         if (KIND == BV_KIND_GENERAL)
             for (int i = tid; i < BV_CHEB_SIZE; i += 128) s_cheb[i] = E.kp.cheb[i];
+        s_etab[tid] = bv_exp_tab_dev[tid]; /* 128 threads, 128 entries */
     }
     __syncthreads();
     const bv_kparams &kp = s_kp;
@@ -1478,7 +1482,8 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
            dynamically (scratch memory) */
         auto gen = [&](int i, int c) {
             return -bv_pair_tile<KIND>(ws, T, myr + i, c0 + c, m, (2 * w + i) * BV_TILE,
-                                       64 + c * BV_TILE, lane, qx, qy, qz, qo, kp, planar, s_cheb);
+                                       64 + c * BV_TILE, lane, qx, qy, qz, qo, kp, planar, s_cheb,
+                                       s_etab);
         };
         bv_d4 acc[2][4] = {{gen(0, 0), gen(0, 1), gen(0, 2), gen(0, 3)},
                            {gen(1, 0), gen(1, 1), gen(1, 2), gen(1, 3)}};
@@ -1558,11 +1563,11 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
     bv_d4 e[2][2];                            /* wave B: D_BA */
     if (w == 0) {
         bv_d4 da0 = d00 - bv_pair_tile<KIND>(ws, T, qn, qn, m, 0, 0, lane, qx, qy, qz, qo, kp,
-                                             planar, s_cheb);
+                                             planar, s_cheb, s_etab);
         bv_d4 da1 = d10 - bv_pair_tile<KIND>(ws, T, qn + 1, qn, m, 16, 0, lane, qx, qy, qz, qo, kp,
-                                             planar, s_cheb);
+                                             planar, s_cheb, s_etab);
         bv_d4 da2 = d11 - bv_pair_tile<KIND>(ws, T, qn + 1, qn + 1, m, 16, 16, lane, qx, qy, qz, qo,
-                                             kp, planar, s_cheb);
+                                             kp, planar, s_cheb, s_etab);
         if (s >= 0) bv_pair_syrk_own(ws, T, myr, c0, lane, da0, da1, da2);
         *(bv_d4 *)(sW[0] + 0 * 256 + 4 * lane) = -da0;
         *(bv_d4 *)(sW[0] + 1 * 256 + 4 * lane) = -da1;
@@ -1570,11 +1575,11 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
         if (lane == 0) { s_logdet[0] = 0.0; s_dotd[0] = 0.0; s_info[0] = 0; }
     } else if (hasB) {
         db0 = d00 - bv_pair_tile<KIND>(ws, T, qn + 2, qn + 2, m, 32, 32, lane, qx, qy, qz, qo, kp,
-                                       planar, s_cheb);
+                                       planar, s_cheb, s_etab);
         db1 = d10 - bv_pair_tile<KIND>(ws, T, qn + 3, qn + 2, m, 48, 32, lane, qx, qy, qz, qo, kp,
-                                       planar, s_cheb);
+                                       planar, s_cheb, s_etab);
         db2 = d11 - bv_pair_tile<KIND>(ws, T, qn + 3, qn + 3, m, 48, 48, lane, qx, qy, qz, qo, kp,
-                                       planar, s_cheb);
+                                       planar, s_cheb, s_etab);
         if (s >= 0) bv_pair_syrk_own(ws, T, myr, c0, lane, db0, db1, db2);
         if (lane == 0) { s_logdet[1] = 0.0; s_dotd[1] = 0.0; s_info[1] = 0; }
     }
@@ -1596,7 +1601,7 @@ bv_pair_kernel(const bv_step_item *__restrict__ items, int nitems,
             for (int c = 0; c < 2; ++c)
                 e[i][c] = -bv_pair_tile<KIND>(ws, T, qn + 2 + i, qn + c, m, (2 + i) * BV_TILE,
                                               c * BV_TILE, lane, qx, qy, qz, qo, kp, planar,
-                                              s_cheb);
+                                              s_cheb, s_etab);
         }
         if (s >= 0) {
             bv_step_gemm<2, 2>(ws, T, qn / 2, c0, qn + 2, lane, e);
