@@ -132,12 +132,13 @@ __device__ __forceinline__ void bv_panel_gemm(const double *__restrict__ ws, int
 
 /* A(rt, panel j) for the row tiles rt = rtb, rtb + rstep, ..., generated from
    the staged locations and written to the tiles (rt, 2j), (rt, 2j+1) in
-   accumulator layout. */
+   accumulator layout (etab: the exponential's table, in LDS). */
 template <int KIND>
 __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T, int j, int rtb,
                                                   int rstep, int lane, int m, const double *sx,
                                                   const double *sy, const double *sz,
-                                                  const double *so, const bv_kparams &kp) {
+                                                  const double *so, const bv_kparams &kp,
+                                                  const double *etab) {
     const int J0 = j * BV_NB;
     const int rt0 = 2 * j;
     const int lr = lane & 15, lq = lane >> 4;
@@ -155,8 +156,7 @@ __device__ __forceinline__ void bv_panel_generate(double *__restrict__ ws, int T
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
                     const int q = q0 + 4 * g + lq;
-                    v[g] = bv_matern_planar<KIND>(xp, yp, sx[q], sy[q], kp, kp.cheb,
-                                                  bv_exp_tab_dev);
+                    v[g] = bv_matern_planar<KIND>(xp, yp, sx[q], sy[q], kp, kp.cheb, etab);
                 }
             } else {
 #pragma unroll
@@ -481,6 +481,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
     __shared__ double s_logdet, s_dotd;
     __shared__ int s_info;
     __shared__ bv_kparams s_kp; /* read where used: not held in registers */
+    __shared__ double s_etab[2 * BV_EXP_TAB]; /* bv_exp_tab_dev, copied */
 
     const bv_block_desc d = desc[blockIdx.x];
     BV_TRACE_WAVE(0, -1);
@@ -502,6 +503,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
         }
     }
     if (tid0 == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; s_kp = E.kp; }
+    for (int i = tid0; i < 2 * BV_EXP_TAB; i += 64 * NW) s_etab[i] = bv_exp_tab_dev[i];
     __syncthreads();
 
     double *ws = ws_base + d.ws_off;
@@ -523,7 +525,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
         /* phase 0 for the first panel (later panels are generated during the
            previous panel's phase 2) */
         if (KIND != BV_KIND_LOADA && j == 0) {
-            bv_panel_generate<KIND>(ws, T, 0, wave, NW, lane, m, sx, sy, sz, so, kp);
+            bv_panel_generate<KIND>(ws, T, 0, wave, NW, lane, m, sx, sy, sz, so, kp, s_etab);
             __syncthreads();
         }
         for (int gi = 0; gi < ngrp; ++gi) {
@@ -536,7 +538,7 @@ bv_block_kernel(const bv_block_desc *__restrict__ desc, const bv_eval_slot *__re
            the other wavefronts generate the next panel's covariance tiles */
         if (KIND != BV_KIND_LOADA && wave > 0 && j + 1 < P)
             bv_panel_generate<KIND>(ws, T, j + 1, rt0 + 2 + wave - 1, NW - 1, lane, m, sx, sy, sz,
-                                    so, kp);
+                                    so, kp, s_etab);
         if (wave == 0) {
             bv_diag_factor_wave(ws, T, rt0, lane, J0, m, cs0, sCol, sRow, sT, sW, sPiv, &s_dotd,
                                 &s_info);
@@ -608,6 +610,7 @@ bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *_
     __shared__ double s_logdet, s_dotd;
     __shared__ int s_info;
     __shared__ bv_kparams s_kp;
+    __shared__ double s_etab[2 * BV_EXP_TAB]; /* bv_exp_tab_dev, copied */
 
     const bv_block_desc d = desc[blockIdx.x];
     BV_TRACE_WAVE(0, -1);
@@ -626,6 +629,8 @@ bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *_
         }
     }
     if (lane0 == 0) { s_logdet = 0.0; s_dotd = 0.0; s_info = 0; s_kp = E.kp; }
+    s_etab[lane0] = bv_exp_tab_dev[lane0];
+    s_etab[lane0 + 64] = bv_exp_tab_dev[lane0 + 64];
     bv_wave_sync();
 
     double *ws = ws_base + d.ws_off;
@@ -636,7 +641,7 @@ bv_block_kernel_1w(const bv_block_desc *__restrict__ desc, const bv_eval_slot *_
         const int rt0 = 2 * j;
         const int nrt = T - rt0;
         if (KIND != BV_KIND_LOADA)
-            bv_panel_generate<KIND>(ws, T, j, rt0, 1, lane, m, sx, sy, sz, so, kp);
+            bv_panel_generate<KIND>(ws, T, j, rt0, 1, lane, m, sx, sy, sz, so, kp, s_etab);
         const int ngrp = (nrt + BV_NRT - 1) / BV_NRT;
         const int gsz = (nrt + ngrp - 1) / ngrp;
         for (int gi = 0; gi < ngrp; ++gi) {
@@ -787,6 +792,7 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems,
     __shared__ double qx[G * BV_TILE + BV_NB], qy[G * BV_TILE + BV_NB], qz[G * BV_TILE + BV_NB],
         qo[G * BV_TILE + BV_NB];
     __shared__ bv_kparams s_kp;
+    __shared__ double s_etab[2 * BV_EXP_TAB]; /* bv_exp_tab_dev, copied */
 
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
@@ -881,6 +887,8 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems,
             qx[t] = xv; qy[t] = yv; qz[t] = zv; qo[t] = ov;
         }
         if (lane == 0) s_kp = E.kp;
+        s_etab[lane] = bv_exp_tab_dev[lane];
+        s_etab[lane + 64] = bv_exp_tab_dev[lane + 64];
         bv_wave_sync();
     }
     bv_d4 acc[G][2];
@@ -909,7 +917,7 @@ bv_step_kernel(const bv_step_item *__restrict__ items, int nitems,
                         for (int g = 0; g < 4; ++g) {
                             const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
                             v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp,
-                                                          kp.cheb, bv_exp_tab_dev);
+                                                          kp.cheb, s_etab);
                         }
                     } else {
 #pragma unroll
@@ -1039,6 +1047,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
     __shared__ double qx[G * BV_TILE + BV_NB], qy[G * BV_TILE + BV_NB], qz[G * BV_TILE + BV_NB],
         qo[G * BV_TILE + BV_NB];
     __shared__ bv_kparams s_kp;
+    __shared__ double s_etab[2 * BV_EXP_TAB]; /* bv_exp_tab_dev, copied */
 
     if ((int)blockIdx.x >= nitems) return;
     const bv_step_item it = items[blockIdx.x];
@@ -1076,6 +1085,8 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
             qx[t] = xv; qy[t] = yv; qz[t] = zv; qo[t] = ov;
         }
         if (lane == 0) s_kp = E.kp;
+        s_etab[lane] = bv_exp_tab_dev[lane];
+        s_etab[lane + 64] = bv_exp_tab_dev[lane + 64];
         bv_wave_sync();
     }
     const bv_kparams &kp = s_kp;
@@ -1111,7 +1122,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
                         for (int g = 0; g < 4; ++g) {
                             const int iq = G * BV_TILE + ct * BV_TILE + 4 * g + lq;
                             v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp,
-                                                          kp.cheb, bv_exp_tab_dev);
+                                                          kp.cheb, s_etab);
                         }
                     } else {
 #pragma unroll
@@ -1196,7 +1207,7 @@ bv_step2_kernel(const bv_step_item *__restrict__ items, int nitems,
                     for (int g = 0; g < 4; ++g) {
                         const int iq = ct * BV_TILE + 4 * g + lq; /* own rows are the columns */
                         v[g] = bv_matern_planar<KIND>(qx[ip], qy[ip], qx[iq], qy[iq], kp,
-                                                      kp.cheb, bv_exp_tab_dev);
+                                                      kp.cheb, s_etab);
                     }
                 } else {
 #pragma unroll
