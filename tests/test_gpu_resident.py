@@ -98,6 +98,37 @@ def test_resident_builds(n, bc, cs, theta):
     _check(b, theta)
 
 
+def test_resident_many_and_graph_over_builds():
+    """An evaluation whose blocks span several builds (one launch per build,
+    the largest blocks' on a second stream): bv_llh_many equals bv_llh bit
+    for bit (each build's descriptors of k evaluations contiguous), HIP graph
+    replays equal direct launches, and the oracle agrees."""
+    x, y = datagen.uniform_locations(4000, 9)
+    obs = np.random.default_rng(9).standard_normal(4000)
+    b = bv.partition(x, y, obs, 60, 150, seed=0)
+    builds = _builds(b, 150)
+    _say("case: n=4000 bc=60 cs=150 blocks per build %s" % builds)
+    assert len(builds) >= 3
+    thetas = [[1.0, 0.1, 0.5], [1.2, 0.08, 0.5], [0.9, 0.12, 0.5]]
+    eng = bv.Engine(b)
+    try:
+        assert eng.factor_config()[0] == 3
+        eng.set_graph(False)
+        single = [eng.llh(t) for t in thetas]
+        many, st = eng.llh_many(np.array(thetas))
+        assert np.all(st == bv.BV_OK) and list(many) == single, (list(many), single)
+        eng.set_graph(True)
+        for rep in range(3):
+            for i, t in enumerate(thetas):
+                got = eng.llh(t)
+                assert got == single[i], (rep, i, got, single[i])
+    finally:
+        eng.close()
+    ref = oracle.llh(b, thetas[0])
+    _say("  gpu %.17g oracle %.17g rel %.3e" % (single[0], ref, rel(single[0], ref)))
+    assert rel(single[0], ref) <= LLH_RTOL
+
+
 def test_resident_3d_and_dist_scale():
     x, y = datagen.uniform_locations(800, 5)
     z = np.random.default_rng(5).random(800) * 0.3
