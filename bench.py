@@ -253,6 +253,69 @@ def measure_uneven(steps=10, warmup=2):
             "steps": steps, "warmup": warmup, "per_nu": out}
 
 
+def measure_mle_eval(steps=20, warmup=3):
+    """Single evaluations at the maximum-likelihood workload's shape: the
+    MLE's n = 100,000 uniform points and Block Vecchia observations (truth
+    (1.0, 0.08, 0.8), simulated with cs = 400), partitioned into bc = 5,000
+    blocks with cs = 150, evaluated at the MLE's estimate (general nu) with
+    the autotuned form: evaluations/s, latency, the factorization's FP64
+    rate."""
+    import blockvecchia_amd as bv
+    from blockvecchia_amd import datagen, flops
+
+    n, bc, cs = 100000, 5000, 150
+    truth = [1.0, 0.08, 0.8]
+    theta = [0.790758, 0.0691475, 0.800678]
+    x, y = datagen.uniform_locations(n, 0)
+    obs = np.asarray(datagen.sample_vecchia(x, y, *truth, cs=400, bc=bc, seed=0))
+    blocks = bv.partition(x, y, obs, bc, cs, seed=0, device=0)
+    F = flops.eval_flops(blocks.batch_num, cs)
+    eng = bv.Engine(blocks)
+    for _ in range(warmup):
+        eng.llh(theta)
+    lat, kms = [], []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        v = eng.llh(theta)
+        lat.append(time.perf_counter() - t1)
+        kms.append(eng.last_kernel_ms())
+    el = time.perf_counter() - t_start
+    kern = float(np.mean(kms))
+    achieved = F / (kern * 1e-3) / 1e12
+    r = {"workload": "MLE shape: n=100000 uniform, Vecchia-simulated observations (truth "
+                     "(1.0, 0.08, 0.8), cs=400), bc=5000, cs=150, evaluated at the MLE "
+                     "estimate (0.790758, 0.0691475, 0.800678)",
+         "n": n, "bc": bc, "cs": cs, "batch_count": int(blocks.batch_count),
+         "steps": steps, "warmup": warmup, "evals_per_s": steps / el,
+         "ms_per_eval": 1e3 * el / steps,
+         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                        "p99": float(np.percentile(lat, 99) * 1e3)},
+         "llh": v, "factor_config": list(eng.factor_config()),
+         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                      "kernel_ms": kern, "flops_per_eval": F}}
+    eng.close()
+    log("mle-eval: %.3f ms/eval (kernels %.3f ms), %.2f TFLOP/s (%.1f%% of peak), config %s" % (
+        r["ms_per_eval"], kern, achieved, 100 * achieved / FP64_PEAK_TFLOPS, r["factor_config"]))
+    return r
+
+
+def run_mle_eval(args):
+    """--workload mle-eval: single evaluations at the MLE's shape, one JSON
+    line (value: evaluations/s)."""
+    r = measure_mle_eval(args.steps, args.warmup)
+    print(json.dumps({
+        "metric": METRIC, "value": r["evals_per_s"], "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_eval"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": r["workload"], "n": r["n"], "bc": r["bc"], "cs": r["cs"],
+                   "batch_count": r["batch_count"], "parallelism": "single-gpu"},
+        "latency_ms": r["latency_ms"], "llh": r["llh"], "factor_config": r["factor_config"],
+        "roofline": r["roofline"]}))
+
+
 def run_uneven(args):
     """--workload uneven: the uneven-batching configuration alone, one JSON
     line (value: evaluations/s at nu = 3/2)."""
@@ -373,10 +436,12 @@ def main():
                     help="append a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 counter CSVs into the traffic JSON and exit")
-    ap.add_argument("--workload", choices=["perf", "uneven", "mle", "serving"], default="perf",
+    ap.add_argument("--workload", choices=["perf", "uneven", "mle", "mle-eval", "serving"],
+                    default="perf",
                     help="perf: the n=20000 cs=300 evaluation (default); uneven: n=200000 "
                          "clustered, cs=400, nu 3/2 and 5/2; mle: the driver's full "
-                         "maximum-likelihood run at n=100000, general nu")
+                         "maximum-likelihood run at n=100000, general nu; mle-eval: single "
+                         "evaluations at that run's shape and estimate")
     ap.add_argument("--nu", type=float, default=0.5,
                     help="smoothness evaluated on the perf workload (its data stay the "
                          "nu=1/2 draw): 0.5, 1.5, 2.5 closed forms, else the device K_nu")
@@ -400,6 +465,8 @@ def main():
         return run_serving_c5(args)
     if args.workload == "mle":
         return run_mle(args)
+    if args.workload == "mle-eval":
+        return run_mle_eval(args)
 
     import blockvecchia_amd as bv
     from blockvecchia_amd import flops
@@ -511,6 +578,7 @@ def main():
     extra = {}
     if not args.no_extra:
         for key, fn in (("uneven", lambda: measure_uneven(10, 2)), ("mle", measure_mle),
+                        ("mle_eval", lambda: measure_mle_eval(20, 3)),
                         ("serving_c5", lambda: __import__("blockvecchia_amd.serving",
                                                           fromlist=["x"]).benchmark_c5(log=log))):
             t0 = time.perf_counter()
