@@ -808,17 +808,17 @@ static const bv_cfg bv_cfgs[] = {{0, 0, 1, 1}, {1, 0, 1, 1}, {2, 0, 1, 1}, {0, 1
                                  {0, 3, 1, 1}, {0, 3, 2, 2}, {0, 3, 3, 3}, {0, 2, 2, 0},
                                  {0, 3, 2, 0}, {3, 0, 1, 1}};
 
-/* whether a context's blocks fit a configuration's kernels */
-/* Whether the autotuner may keep configuration g.  The register-resident
-   form is offered where it holds every block and some block is above
-   BV_RES_TUNE_MIN_MP: on datasets of only tiny blocks (padded size <= 128) a
-   four-wavefront workgroup per block loses in the request server's merged
-   launches to the one-wavefront fused kernel (small-dataset serving at 0.81x
-   the throughput), although one evaluation alone is faster; it can still be
-   forced (BV_FACTOR_KERNEL=res). */
+/* Whether the autotuner may keep configuration g (a context's blocks fit its
+   kernels).  The register-resident form is offered where it holds every
+   block and some block is above BV_RES_TUNE_MIN_MP: on datasets of only tiny
+   blocks (padded size <= 128) a four-wavefront workgroup per block loses in
+   the request server's merged launches to the one-wavefront fused kernel
+   (small-dataset serving at 0.81x the throughput), although one evaluation
+   alone is faster; it can still be forced (BV_FACTOR_KERNEL=res). */
 static bool bv_cfg_ok(const bv_ctx *c, const bv_cfg &g) {
     if (g.stepped) return true;
-    return g.variant == 3 ? (c->res_ok != 0 && c->max_mp >= BV_RES_TUNE_MIN_MP) : c->fused_ok != 0;
+    if (g.variant == 3) return c->res_ok != 0 && c->max_mp >= BV_RES_TUNE_MIN_MP;
+    return c->fused_ok != 0;
 }
 
 static int bv_apply_cfg(bv_ctx *c, const bv_cfg &g) {
