@@ -86,6 +86,20 @@ __device__ __forceinline__ double bv_res_dist(double xp, double yp, double zp, d
     return bv_sqrt_nr(s);
 }
 
+/* The covariance at distance d: the closed forms of bv_matern_closed with
+   the table exponential, or general nu's entries of bv_matern_planar, with
+   the tables in LDS. */
+template <int KIND>
+__device__ __forceinline__ double bv_res_entry(double d, const bv_res_block &B,
+                                               const bv_kparams &kp) {
+    const double t = d * kp.inv_scale;
+    if (KIND == BV_KIND_GENERAL) return bv_matern_general_x(t, kp, B.cheb, B.etab);
+    const double x = bv_expneg_tab(t, B.etab);
+    if (KIND == BV_KIND_NU05) return kp.sigma2 * x;
+    if (KIND == BV_KIND_NU15) return kp.sigma2 * (1.0 + t) * x;
+    return kp.sigma2 * (1.0 + t + t * t * (1.0 / 3.0)) * x;
+}
+
 /* A(r, c) in accumulator layout (lane: row 16 r + lane % 16, columns
    16 c + 4 g + lane / 16).  Entries outside the block (rows or columns >= m)
    follow the augmented matrix: row m holds the observations, the padding is
@@ -105,22 +119,22 @@ __device__ __forceinline__ bv_d4 bv_res_gen(const bv_res_block &B, int r, int c,
     } else {
         const double xp = B.sx[p], yp = B.sy[p], zp = B.sz[p];
         const bool use_z = kp.use_z != 0;
+        if (!use_z && 16 * r + 15 < B.m && 16 * c + 15 < B.m) {
+            /* a tile inside the block, planar distances (wave-uniform): the
+               same entries without the augmented matrix's selects */
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int q = 16 * c + 4 * g + lq;
+                v[g] = bv_res_entry<KIND>(bv_res_dist(xp, yp, 0.0, B.sx[q], B.sy[q], 0.0, false),
+                                          B, kp);
+            }
+            return v;
+        }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int q = 16 * c + 4 * g + lq;
             const double d = bv_res_dist(xp, yp, zp, B.sx[q], B.sy[q], B.sz[q], use_z);
-            double e;
-            if (KIND == BV_KIND_GENERAL) {
-                /* the entries of bv_matern_planar, with the tables in LDS */
-                e = bv_matern_general_x(d * kp.inv_scale, kp, B.cheb, B.etab);
-            } else {
-                /* the closed forms of bv_matern_closed with the table exponential */
-                const double t = d * kp.inv_scale;
-                const double x = bv_expneg_tab(t, B.etab);
-                if (KIND == BV_KIND_NU05) e = kp.sigma2 * x;
-                else if (KIND == BV_KIND_NU15) e = kp.sigma2 * (1.0 + t) * x;
-                else e = kp.sigma2 * (1.0 + t + t * t * (1.0 / 3.0)) * x;
-            }
+            const double e = bv_res_entry<KIND>(d, B, kp);
             const double pad = (p == B.m && q < B.m) ? B.so[q] : ((p == q) ? 1.0 : 0.0);
             v[g] = (p < B.m && q < B.m) ? e : pad;
         }
