@@ -297,17 +297,18 @@ BV_HD double bv_xnu_knu_tab(double x, const bv_knu_tab &t) {
 }
 
 /* ---- x^nu K_nu(x) from a per-nu Chebyshev table --------------------------
- * g(x) = x^nu K_nu(x) e^x is smooth on every half-octave: on
- * I(e, h) = 2^e [lo_h, hi_h), lo_0 = 1, hi_0 = lo_1 = sqrt 2, hi_1 = 2, for
- * e in [BV_CHEB_EMIN, BV_CHEB_EMAX), it is kept as a Chebyshev series of
- * BV_CHEB_N terms in t = (u - mid_h) / hw_h, u = x 2^-e (so t in [-1, 1]),
- * whose coefficients come from the accurate routine (bv_besselk_scaled) at
- * the Chebyshev points of each half-octave (bv_cheb_node_x; interpolation in
- * those points, coefficients by the discrete cosine sums, bv_cheb_build_kernel).  An entry then costs the
- * Clenshaw recurrence and one exp: f = g e^-x.  Interpolation error (against
- * 30-digit mpmath, nu in [0.01, 2.7], every half-octave): below 5e-15
- * relative; below 2^EMIN the accurate routine runs, from 2^EMAX on
- * f = 0 (e^-x is below the smallest double). */
+ * f(x) = x^nu K_nu(x) and g(x) = f(x) e^x are smooth on every half-octave:
+ * on I(e, h) = 2^e [lo_h, hi_h), lo_0 = 1, hi_0 = lo_1 = sqrt 2, hi_1 = 2,
+ * for e in [BV_CHEB_EMIN, BV_CHEB_EMAX), f (e < 0, x < 1) or g (e >= 0) is
+ * kept as a Chebyshev series of BV_CHEB_N terms in t = (u - mid_h) / hw_h,
+ * u = x 2^-e (so t in [-1, 1]), whose coefficients come from the accurate
+ * routine (bv_besselk, bv_besselk_scaled) at the Chebyshev points of each
+ * half-octave (bv_cheb_node_x; interpolation in those points, coefficients
+ * by the discrete cosine sums, bv_cheb_build_kernel).  An entry then costs
+ * the Clenshaw recurrence, and from x = 1 on one exponential: f = g e^-x.
+ * Interpolation error (against 30-digit mpmath, nu in [0.01, 2.7], every
+ * half-octave): below 5e-15 relative; below 2^EMIN the accurate routine
+ * runs, from 2^EMAX on f = 0 (e^-x is below the smallest double). */
 #define BV_CHEB_N 14
 #define BV_CHEB_EMIN (-24)
 #define BV_CHEB_EMAX 10
@@ -337,9 +338,13 @@ BV_HD double bv_cheb_node_x(int j, int k) {
     return ldexp(u, e);
 }
 
-/* The table's input: g = x^nu e^x K_nu(x) at node k of half-octave j. */
+/* The table's input at node k of half-octave j: below x = 1 (exponent < 0)
+   f = x^nu K_nu(x) itself, from 1 on g = x^nu e^x K_nu(x) (f falls like
+   e^-x there, which no short series on a half-octave follows in relative
+   terms; below 1, e^-x is as smooth as f and needs no separate factor). */
 BV_HD double bv_cheb_node_g(double nu, int j, int k) {
     const double x = bv_cheb_node_x(j, k);
+    if (BV_CHEB_EMIN + (j >> 1) < 0) return exp(nu * log(x)) * bv_besselk(nu, x);
     return exp(nu * log(x)) * bv_besselk_scaled(nu, x);
 }
 
@@ -358,11 +363,14 @@ BV_HD double bv_cheb_g(double x, const double *cheb);
 BV_HD double bv_xnu_knu_cheb(double x, const double *cheb, double nu) {
     if (!(x >= 0x1p-24)) return bv_xnu_knu_slow(x, nu);
     if (x >= 0x1p10) return 0.0;
-    return bv_cheb_g(x, cheb) * exp(-x);
+    const double g = bv_cheb_g(x, cheb);
+    if (x < 1.0) return g; /* the table holds x^nu K_nu(x) there */
+    return g * exp(-x);
 }
 
-/* g(x) = x^nu e^x K_nu(x) for x in [2^-24, 2^10) from the table: the
-   half-octave from the exponent bits, Clenshaw's recurrence on it. */
+/* The table's function at x in [2^-24, 2^10) (x^nu K_nu(x) below 1,
+   x^nu e^x K_nu(x) from 1 on): the half-octave from the exponent bits,
+   Clenshaw's recurrence on it. */
 BV_HD double bv_cheb_g(double x, const double *cheb) {
     uint64_t b;
     __builtin_memcpy(&b, &x, sizeof(b));
@@ -455,7 +463,9 @@ BV_HD double bv_expneg_tab(double e, const double *tab) {
 BV_HD double bv_xnu_knu_cheb_t(double x, const double *cheb, double nu, const double *tab) {
     if (!(x >= 0x1p-24)) return bv_xnu_knu_slow(x, nu);
     if (x >= 0x1p10) return 0.0;
-    return bv_cheb_g(x, cheb) * bv_expneg_tab(x, tab);
+    const double g = bv_cheb_g(x, cheb);
+    if (x < 1.0) return g; /* the table holds x^nu K_nu(x) there */
+    return g * bv_expneg_tab(x, tab);
 }
 
 /* General-nu entry at the scaled distance x = d * inv_scale (the reciprocal
