@@ -129,6 +129,31 @@ def test_resident_many_and_graph_over_builds():
     assert rel(single[0], ref) <= LLH_RTOL
 
 
+def test_resident_not_tuned_for_tiny_blocks(monkeypatch):
+    """The autotuner does not keep the resident form on a dataset whose blocks
+    are all at most 128 rows padded (merged serving is faster with the
+    one-wavefront fused kernel there); forced, it still runs and agrees with
+    the oracle."""
+    monkeypatch.delenv("BV_FACTOR_KERNEL", raising=False)
+    monkeypatch.setenv("BV_TUNE_CACHE", "0")
+    x, y = datagen.uniform_locations(1500, 10)
+    obs = np.random.default_rng(10).standard_normal(1500)
+    b = bv.partition(x, y, obs, 100, 40, seed=0)
+    assert _max_mp(b, 40) <= 128
+    eng = bv.Engine(b)
+    try:
+        v = eng.llh([1.0, 0.1, 0.5])
+        cfg = eng.factor_config()
+    finally:
+        eng.close()
+    _say("case: tiny blocks (padded <= %d): autotuned %s" % (_max_mp(b, 40), cfg))
+    assert cfg[0] != 3
+    ref = oracle.llh(b, [1.0, 0.1, 0.5])
+    assert rel(v, ref) <= LLH_RTOL
+    monkeypatch.setenv("BV_FACTOR_KERNEL", "res")
+    _check(b, [1.0, 0.1, 0.5])
+
+
 def test_resident_3d_and_dist_scale():
     x, y = datagen.uniform_locations(800, 5)
     z = np.random.default_rng(5).random(800) * 0.3
