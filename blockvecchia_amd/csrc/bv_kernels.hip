@@ -344,7 +344,8 @@ __device__ __forceinline__ bv_d4 bv_sweep16(bv_d4 a, int lane, int kbase, int m,
     for (int g = 0; g < 4; ++g) w0[g] = (4 * g + lq0 == lr0) ? 1.0 : 0.0;
     bv_sweep_steps<0>(a, w0, lane, kbase, m, cs0, sPiv, pivoff, dacc, info);
     return w0;
-#endif
+#else
+    /* the same sweep with the column factors and W's row K through LDS */
     const int lr = lane & 15, lq = lane >> 4;
     const int kml = m - kbase;
     bv_d4 w;
@@ -389,6 +390,7 @@ __device__ __forceinline__ bv_d4 bv_sweep16(bv_d4 a, int lane, int kbase, int m,
         bv_wave_sync(); /* reads done before the next step's writes */
     }
     return w;
+#endif
 }
 
 /* Log-determinant terms of one panel: log(l_kk^2) for the block rows
