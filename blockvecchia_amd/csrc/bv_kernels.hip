@@ -1796,18 +1796,27 @@ hipError_t bv_launch_sim_finish(unsigned nblk, hipStream_t st, const bv_block_de
 
 /* The general-nu table of every evaluation slot of a call (bv_xnu_knu_cheb):
    one workgroup per (half-octave, slot); slots of other kinds are left
-   alone.  The node values g = x^nu e^x K_nu(x) come from the accurate
-   routine, the coefficients from their cosine sums.  Written through
-   kp.cheb, the slot's own table. */
+   alone.  The node values (bv_cheb_node_g) come from the accurate routine,
+   the Chebyshev coefficients from their cosine sums, the monomial ones from
+   those (bv_cheb_mono).  Written through kp.cheb, the slot's own table. */
 extern "C" __global__ void __launch_bounds__(64)
 bv_cheb_build_kernel(const bv_eval_slot *__restrict__ slots) {
     const bv_eval_slot &E = slots[blockIdx.y];
     if (E.kind != BV_KIND_GENERAL) return;
-    __shared__ double gv[BV_CHEB_N];
+    __shared__ double gv[BV_CHEB_N], cv[BV_CHEB_N];
     const int j = blockIdx.x, t = threadIdx.x;
     if (t < BV_CHEB_N) gv[t] = bv_cheb_node_g(E.kp.nu, j, t);
     __syncthreads();
-    if (t < BV_CHEB_N) ((double *)E.kp.cheb)[j * BV_CHEB_N + t] = bv_cheb_coef(gv, t);
+    if (t < BV_CHEB_N) cv[t] = bv_cheb_coef(gv, t);
+    __syncthreads();
+    if (t == 0) {
+        double a[BV_CHEB_N];
+#pragma unroll
+        for (int i = 0; i < BV_CHEB_N; ++i) a[i] = cv[i];
+        bv_cheb_mono(a);
+#pragma unroll
+        for (int i = 0; i < BV_CHEB_N; ++i) ((double *)E.kp.cheb)[j * BV_CHEB_N + i] = a[i];
+    }
 }
 
 hipError_t bv_launch_cheb_build(unsigned nslots, hipStream_t st, const bv_eval_slot *slots) {

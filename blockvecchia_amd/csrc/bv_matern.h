@@ -304,11 +304,15 @@ BV_HD double bv_xnu_knu_tab(double x, const bv_knu_tab &t) {
  * u = x 2^-e (so t in [-1, 1]), whose coefficients come from the accurate
  * routine (bv_besselk, bv_besselk_scaled) at the Chebyshev points of each
  * half-octave (bv_cheb_node_x; interpolation in those points, coefficients
- * by the discrete cosine sums, bv_cheb_build_kernel).  An entry then costs
- * the Clenshaw recurrence, and from x = 1 on one exponential: f = g e^-x.
- * Interpolation error (against 30-digit mpmath, nu in [0.01, 2.7], every
- * half-octave): below 5e-15 relative; below 2^EMIN the accurate routine
- * runs, from 2^EMAX on f = 0 (e^-x is below the smallest double). */
+ * by the discrete cosine sums, bv_cheb_build_kernel), stored as the same
+ * polynomial's monomial coefficients in t (bv_cheb_mono).  An entry then
+ * costs Horner's rule (13 multiply-adds), and from x = 1 on one exponential:
+ * f = g e^-x.
+ * Error against 30-digit mpmath (nu in [0.01, 2.7], 11 points in every
+ * half-octave): 7.5e-15 relative at worst, the interpolation's own (the
+ * Chebyshev form evaluated by Clenshaw's recurrence has the same worst
+ * case); below 2^EMIN the accurate routine runs, from 2^EMAX on f = 0 (e^-x
+ * is below the smallest double). */
 #define BV_CHEB_N 14
 #define BV_CHEB_EMIN (-24)
 #define BV_CHEB_EMAX 10
@@ -357,6 +361,33 @@ BV_HD double bv_cheb_coef(const double *gv, int t) {
     return (t == 0) ? 0.5 * sum : sum;
 }
 
+/* In place: the Chebyshev coefficients c[0..BV_CHEB_N) of a half-octave's
+   series (constant term already halved) become the monomial coefficients of
+   the same polynomial in t, by Clenshaw's recurrence run on coefficient
+   vectors (b_k = 2 t b_(k+1) - b_(k+2) + c_k, the polynomial t b_1 - b_2 +
+   c_0).  Well conditioned here: |c_k| falls by about a factor of 10 per term,
+   faster than the 2^(k-1) growth of T_k's coefficients. */
+BV_HD void bv_cheb_mono(double *c) {
+    double b1[BV_CHEB_N], b2[BV_CHEB_N];
+#pragma unroll
+    for (int i = 0; i < BV_CHEB_N; ++i) b1[i] = b2[i] = 0.0;
+#pragma unroll
+    for (int k = BV_CHEB_N - 1; k >= 1; --k) {
+        double b0[BV_CHEB_N];
+#pragma unroll
+        for (int i = 0; i < BV_CHEB_N; ++i)
+            b0[i] = ((i > 0) ? 2.0 * b1[i - 1] : 0.0) - b2[i] + ((i == 0) ? c[k] : 0.0);
+#pragma unroll
+        for (int i = 0; i < BV_CHEB_N; ++i) {
+            b2[i] = b1[i];
+            b1[i] = b0[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < BV_CHEB_N; ++i)
+        c[i] = ((i > 0) ? b1[i - 1] : 0.0) - b2[i] + ((i == 0) ? c[0] : 0.0);
+}
+
 /* x^nu K_nu(x), x > 0, from the table. */
 BV_HD double bv_cheb_g(double x, const double *cheb);
 
@@ -370,7 +401,7 @@ BV_HD double bv_xnu_knu_cheb(double x, const double *cheb, double nu) {
 
 /* The table's function at x in [2^-24, 2^10) (x^nu K_nu(x) below 1,
    x^nu e^x K_nu(x) from 1 on): the half-octave from the exponent bits,
-   Clenshaw's recurrence on it. */
+   Horner's rule on its monomial coefficients. */
 BV_HD double bv_cheb_g(double x, const double *cheb) {
     uint64_t b;
     __builtin_memcpy(&b, &x, sizeof(b));
@@ -382,15 +413,10 @@ BV_HD double bv_cheb_g(double x, const double *cheb) {
     const double t = h ? (u - BV_CHEB_MID1) * (1.0 / BV_CHEB_HW1)
                        : (u - BV_CHEB_MID0) * (1.0 / BV_CHEB_HW0);
     const double *c = cheb + (2 * (e - BV_CHEB_EMIN) + h) * BV_CHEB_N;
-    const double t2 = 2.0 * t;
-    double b1 = 0.0, b2 = 0.0;
+    double p = c[BV_CHEB_N - 1];
 #pragma unroll
-    for (int k = BV_CHEB_N - 1; k >= 1; --k) {
-        const double b0 = t2 * b1 - b2 + c[k];
-        b2 = b1;
-        b1 = b0;
-    }
-    return t * b1 - b2 + c[0];
+    for (int k = BV_CHEB_N - 2; k >= 0; --k) p = fma(p, t, c[k]);
+    return p;
 }
 
 /* Great-circle distance (src/ckernel.cpp:145-154). */

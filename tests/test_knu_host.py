@@ -28,6 +28,7 @@ extern "C" void shim_cheb_table(double nu, double *tab) {
     for (int j = 0; j < BV_CHEB_NJ; ++j) {
         for (int k = 0; k < BV_CHEB_N; ++k) gv[k] = bv_cheb_node_g(nu, j, k);
         for (int t = 0; t < BV_CHEB_N; ++t) tab[j * BV_CHEB_N + t] = bv_cheb_coef(gv, t);
+        bv_cheb_mono(tab + j * BV_CHEB_N);
     }
 }
 extern "C" double shim_xnu_knu_cheb(double nu, double x, const double *tab) {
