@@ -13,7 +13,11 @@ line with the driver's contract fields plus `roofline` (the factorization
 kernel's FP64 rate, HIP-event timed) and `cpu_baseline` (the CPU port of the
 same chain, timed on this host's cores).
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 50] [--no-cpu-baseline]
+
+The defaults time the steady state: with 3 to 5 warm-up evaluations the
+first tens of timed evaluations on a fresh process run up to 8% slower
+(0.76-0.81 ms against 0.74-0.75 ms per evaluation on one box).
 
 HBM traffic of the factorization launches (the `traffic` field, per
 evaluation) comes from two rocprofv3 counter passes over this script with the
@@ -422,8 +426,8 @@ def run_mle(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--no-serving", action="store_true",
