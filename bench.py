@@ -15,9 +15,9 @@ same chain, timed on this host's cores).
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 50] [--no-cpu-baseline]
 
-The defaults time the steady state: with 3 to 5 warm-up evaluations the
-first tens of timed evaluations on a fresh process run up to 8% slower
-(0.76-0.81 ms against 0.74-0.75 ms per evaluation on one box).
+The defaults average over 200 evaluations after 50 warm-up: 20-evaluation
+windows of separate processes on one box spread from 0.76 to 0.81 ms per
+evaluation, 200-evaluation ones from 0.74 to 0.75 ms.
 
 HBM traffic of the factorization launches (the `traffic` field, per
 evaluation) comes from two rocprofv3 counter passes over this script with the
